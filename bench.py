@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""RK3 steps/s of the compressible Hall-MHD step at 256^3 on one MI355X.
+
+Workload (the 256^3 Hall-MHD case): periodic box [0,2pi)^3, gamma = 5/3,
+rho0 = 1, P0 = 1, B0 = (1,0,0), seeded random-phase modes in shells 1-16
+with amplitude |k|^(-5/3), rms |db| = 0.3 B0 and rms |du| = 0.3, d_i = 0.05,
+nu = eta = 2e-4 (implicit), 2/3-rule dealiasing (the reference's option 1),
+CFL 0.4. A step is one full RK3 step (three substages, CFL dt on the device).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line on stdout: `value` is FP64 steps/s with the state
+resident in HBM; the mixed-precision run (FP32 transforms and nonlinear
+products, FP64 RK3 accumulation) is reported under "mixed" with its relative
+L2 distance to the FP64 run after the same number of steps. `roofline` is
+the dominant kernel's algorithmic bytes over its HIP-event time; `cpu_baseline`
+is the CPU oracle timed on this host for one step of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload_params(n, precision):
+    from laps_amd.params import default_params
+    return default_params(nx=n, ny=n, nz=n, Lx=2 * math.pi, Ly=2 * math.pi, Lz=2 * math.pi,
+                          adiabatic_index=5.0 / 3.0, cfl=0.4, dealias_option=1,
+                          if_visc=1, viscosity=2e-4, if_resis=1, resistivity=2e-4,
+                          if_hall=1, ion_inertial_length=0.05, precision=precision)
+
+
+def initial_state(n, gamma):
+    from laps_amd.ic import cons_from_prim, seeded_primitive
+    prim = seeded_primitive((n, n, n), seed=1, kmin=1, kmax=16, du=0.3, db=0.3)
+    return cons_from_prim(prim, gamma)
+
+
+def shell_spectrum(uhat, n):
+    """kinetic (rho u)^2/2 + magnetic B^2/2 energy per integer shell, Hermitian-weighted"""
+    nxh = uhat.shape[0]
+    mx = np.arange(nxh)[:, None, None]
+    m = np.fft.fftfreq(n, 1.0 / n)
+    kk = np.sqrt(mx ** 2 + m[None, :, None] ** 2 + m[None, None, :] ** 2)
+    w = np.where(mx == 0, 1.0, 2.0) * np.ones_like(kk)
+    e = sum(np.abs(uhat[..., v]) ** 2 for v in (1, 2, 3, 4, 5, 6)) * 0.5 * w
+    shell = np.rint(kk).astype(np.int64).ravel()
+    return np.bincount(shell, weights=e.ravel())
+
+
+def run_gpu(n, precision, steps, warmup, uu, pr):
+    from laps_amd.runtime import Laps
+    p = workload_params(n, precision)
+    g = Laps(p)
+    g.set_state(uu, pr)
+    log(f"precision={'fp64' if precision == 0 else 'mixed'}: warmup {warmup} steps")
+    g.step(warmup)
+    g.sync()
+    g.timing_enable(True)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    timing = g.timing()
+    g.timing_enable(False)
+    sc = g.scalars()
+    spec = g.spectral()
+    g.close()
+    log(f"  {steps} steps in {t1 - t0:.3f} s -> {steps / (t1 - t0):.2f} steps/s, t={sc['time']:.5f} dt={sc['dt']:.3e}")
+    return dict(seconds=t1 - t0, timing=timing, scalars=sc, spectral=spec)
+
+
+def roofline(timing):
+    name, (ms, by, nl) = max(timing.items(), key=lambda kv: kv[1][0])
+    per_launch_s = ms * 1e-3 / max(nl, 1)
+    achieved = (by / max(nl, 1)) / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            t = json.load(open(tfile))
+            if t.get("kernel") == name:
+                traffic = t.get("bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+    return dict(kernel=name, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                algorithmic_bytes_per_launch=by / max(nl, 1), avg_launch_us=round(per_launch_s * 1e6, 2),
+                launches=nl)
+
+
+def cpu_baseline(n, uu, pr, threads):
+    from oracle import binding as ob
+    ob.set_threads(threads)
+    p = workload_params(n, 0)
+    o = ob.Oracle(p)
+    log(f"cpu baseline: oracle set_state at {n}^3 on {threads} threads")
+    o.set_state(uu, pr)
+    t0 = time.perf_counter()
+    o.step(1)
+    dt = time.perf_counter() - t0
+    o.close()
+    log(f"cpu baseline: 1 step in {dt:.2f} s")
+    return dict(value=round(1.0 / dt, 5), unit="steps/s", cores=threads, kind="port",
+                sample=f"1 RK3 step of the same {n}^3 Hall-MHD workload (FP64 C oracle, OpenMP)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the box would shard as slabs with one all-to-all per 3D transform; not built).",
+              file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    from laps_amd.params import LAPS_FP64, LAPS_MIXED
+    log(f"building {n}^3 seeded initial condition")
+    uu, pr = initial_state(n, 5.0 / 3.0)
+    res = {}
+    if args.precision in ("fp64", "both"):
+        res["fp64"] = run_gpu(n, LAPS_FP64, args.steps, args.warmup, uu, pr)
+    if args.precision in ("mixed", "both"):
+        res["mixed"] = run_gpu(n, LAPS_MIXED, args.steps, args.warmup, uu, pr)
+    main_key = "fp64" if "fp64" in res else "mixed"
+    r = res[main_key]
+    sps = args.steps / r["seconds"]
+    out = {
+        "metric": "RK3 steps/s at 256^3 Hall-MHD, FP64 vs mixed; % of HBM roofline",
+        "value": round(sps, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * r["seconds"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_key == "fp64" else "f32/f64",
+        "data": "synthetic (seeded k^-5/3 shells 1-16 initial condition)",
+        "config": {"workload": f"compressible Hall-MHD RK3 step, {n}^3 periodic, d_i=0.05, nu=eta=2e-4, "
+                               f"2/3-rule dealiasing, CFL 0.4",
+                   "grid": [n, n, n], "precision": main_key, "dealias_option": 1},
+        "roofline": roofline(r["timing"]),
+        "kernels": {k: dict(ms_per_step=round(v[0] / args.steps, 4), gbs=round(v[1] / (v[0] * 1e-3) / 1e9, 1)
+                            if v[0] > 0 else 0.0, launches=v[2]) for k, v in r["timing"].items() if v[2]},
+    }
+    if "fp64" in res and "mixed" in res:
+        m = res["mixed"]
+        a, b = m["spectral"], res["fp64"]["spectral"]
+        rel = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+        ea, eb = shell_spectrum(a, n), shell_spectrum(b, n)
+        kmax = n // 3
+        sel = np.arange(len(eb)) <= kmax
+        sel &= eb > 1e-10 * eb[1:].max()      # shells holding energy above round-off level
+        sel[0] = False
+        dev = float(np.max(np.abs(ea[sel] - eb[sel]) / eb[sel]))
+        msps = args.steps / m["seconds"]
+        out["mixed"] = dict(value=round(msps, 3), unit="steps/s", speedup_vs_fp64=round(msps / sps, 3),
+                            rel_l2_vs_fp64=rel, steps_compared=args.steps + args.warmup,
+                            max_shell_energy_dev=dev, roofline=roofline(m["timing"]),
+                            ms_per_step=round(1000.0 * m["seconds"] / args.steps, 4))
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            out["cpu_baseline"] = cpu_baseline(n, uu, pr, threads)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e}")
+            out["cpu_baseline"] = None
+    for v in res.values():
+        v.pop("spectral", None)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+// fft_dev.hpp -- device-side FFT building blocks for gfx950 (CDNA4).
+//
+// A 1D transform of length N is factored into 2-3 Stockham stages of radix
+// R_s (register butterflies). One work-item owns butterfly j of a line; the
+// first stage reads its operands straight from HBM (positions j + r*N/R0),
+// the last stage leaves its results in registers at positions j + r*N/RL,
+// and the stages in between exchange data through LDS. A 256-point line
+// therefore costs one LDS round trip. Twiddles W_N^t come from a table
+// computed in FP64 on the host (rounded once for FP32 kernels).
+//
+// Sign convention (fftw.f90:11): forward DIR = -1, inverse DIR = +1,
+// both unnormalised.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace laps {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+// multiply by exp(DIR * i pi / 2): DIR=-1 -> -i, DIR=+1 -> +i
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> rotq(cpx<T> a) {
+    if constexpr (DIR < 0) return {a.y, -a.x};
+    else return {-a.y, a.x};
+}
+
+// cos / sin of 2*pi*m/16, m = 0..15
+__device__ constexpr double kCos16[16] = {
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+__device__ constexpr double kSin16[16] = {
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+
+// W_R^m = exp(DIR * 2 pi i m / R) for R | 16
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cpx<T> wconst(int m) {
+    const int i = (m * (16 / R)) & 15;
+    return {T(kCos16[i]), T(DIR * kSin16[i])};
+}
+
+template <int R, int DIR, typename T>
+struct Dft;
+
+template <int DIR, typename T>
+struct Dft<2, DIR, T> {
+    __device__ __forceinline__ static void run(cpx<T>* v) {
+        cpx<T> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    }
+};
+
+template <int DIR, typename T>
+struct Dft<4, DIR, T> {
+    __device__ __forceinline__ static void run(cpx<T>* v) {
+        cpx<T> t0 = v[0] + v[2], t1 = v[0] - v[2];
+        cpx<T> t2 = v[1] + v[3], t3 = rotq<DIR>(v[1] - v[3]);
+        v[0] = t0 + t2;
+        v[2] = t0 - t2;
+        v[1] = t1 + t3;
+        v[3] = t1 - t3;
+    }
+};
+
+// R = A*B in registers: n = B*n1 + n2, k = k1 + A*k2.
+template <int A, int B, int DIR, typename T>
+__device__ __forceinline__ void dft_split(cpx<T>* v) {
+    constexpr int R = A * B;
+    cpx<T> y[R];  // y[n2*A + k1]
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+        cpx<T> t[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) t[n1] = v[B * n1 + n2];
+        Dft<A, DIR, T>::run(t);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) {
+            cpx<T> w = (n2 * k1 == 0) ? cpx<T>{T(1), T(0)} : wconst<R, DIR, T>(n2 * k1);
+            y[n2 * A + k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], w);
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        cpx<T> t[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) t[n2] = y[n2 * A + k1];
+        Dft<B, DIR, T>::run(t);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = t[k2];
+    }
+}
+
+template <int DIR, typename T>
+struct Dft<8, DIR, T> {
+    __device__ __forceinline__ static void run(cpx<T>* v) { dft_split<2, 4, DIR, T>(v); }
+};
+
+template <int DIR, typename T>
+struct Dft<16, DIR, T> {
+    __device__ __forceinline__ static void run(cpx<T>* v) { dft_split<4, 4, DIR, T>(v); }
+};
+
+// Stage plans: radices R0, R1[, R2]. TPL = work-items per line (max butterflies).
+template <int N>
+struct Plan;
+template <> struct Plan<16>  { static constexpr int NST = 2, R0 = 4,  R1 = 4,  R2 = 1; };
+template <> struct Plan<32>  { static constexpr int NST = 2, R0 = 8,  R1 = 4,  R2 = 1; };
+template <> struct Plan<64>  { static constexpr int NST = 2, R0 = 8,  R1 = 8,  R2 = 1; };
+template <> struct Plan<128> { static constexpr int NST = 2, R0 = 16, R1 = 8,  R2 = 1; };
+template <> struct Plan<256> { static constexpr int NST = 2, R0 = 16, R1 = 16, R2 = 1; };
+template <> struct Plan<512> { static constexpr int NST = 3, R0 = 8,  R1 = 8,  R2 = 8; };
+
+template <int N>
+struct PlanInfo {
+    using P = Plan<N>;
+    static constexpr int B0 = N / P::R0;
+    static constexpr int B1 = N / P::R1;
+    static constexpr int B2 = P::NST == 3 ? N / P::R2 : 0;
+    static constexpr int TPL = B0 > B1 ? (B0 > B2 ? B0 : B2) : (B1 > B2 ? B1 : B2);
+    static constexpr int RMAX = P::R0 > P::R1 ? (P::R0 > P::R2 ? P::R0 : P::R2) : (P::R1 > P::R2 ? P::R1 : P::R2);
+    static constexpr int RLAST = P::NST == 3 ? P::R2 : P::R1;
+    static constexpr int BLAST = N / RLAST;
+};
+
+// twiddle W_N^t (table holds the forward roots exp(-2 pi i t/N))
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> twiddle(const cpx<T>* __restrict__ tw, int t) {
+    cpx<T> w = tw[t];
+    if constexpr (DIR > 0) w.y = -w.y;
+    return w;
+}
+
+// One radix-R butterfly of the Stockham stage with span NS, butterfly j.
+template <int N, int R, int NS, int DIR, typename T>
+__device__ __forceinline__ void stage_bfly(cpx<T>* v, int j, const cpx<T>* __restrict__ tw) {
+    if constexpr (NS > 1) {
+        const int k = j % NS;
+        constexpr int STEP = N / (NS * R);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<DIR>(tw, r * k * STEP));
+    }
+    Dft<R, DIR, T>::run(v);
+}
+
+template <int NS, int R>
+__device__ __forceinline__ int stage_out(int j, int r) {
+    return (j / NS) * NS * R + (j % NS) + r * NS;
+}
+
+// Full line transform up to the last stage's butterfly. `load(p)` supplies
+// input position p and `lds(p)` is the LDS address of position p of this
+// work-item's line. On return, if j < BLAST, v[r] holds output position
+// j + r*BLAST (r < RLAST). Every work-item of the block must call this.
+template <int N, int DIR, typename T, class Load, class Lds>
+__device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds,
+                                              cpx<T>* v) {
+    using P = Plan<N>;
+    using I = PlanInfo<N>;
+    // stage 0 (span 1): HBM -> registers -> LDS
+    if (j < I::B0) {
+#pragma unroll
+        for (int r = 0; r < P::R0; ++r) v[r] = load(j + r * I::B0);
+        stage_bfly<N, P::R0, 1, DIR, T>(v, j, tw);
+#pragma unroll
+        for (int r = 0; r < P::R0; ++r) *lds(stage_out<1, P::R0>(j, r)) = v[r];
+    }
+    __syncthreads();
+    if constexpr (P::NST == 3) {
+        constexpr int NS1 = P::R0;
+        if (j < I::B1) {
+#pragma unroll
+            for (int r = 0; r < P::R1; ++r) v[r] = *lds(j + r * I::B1);
+            stage_bfly<N, P::R1, NS1, DIR, T>(v, j, tw);
+        }
+        __syncthreads();
+        if (j < I::B1) {
+#pragma unroll
+            for (int r = 0; r < P::R1; ++r) *lds(stage_out<NS1, P::R1>(j, r)) = v[r];
+        }
+        __syncthreads();
+        constexpr int NS2 = P::R0 * P::R1;
+        if (j < I::B2) {
+#pragma unroll
+            for (int r = 0; r < P::R2; ++r) v[r] = *lds(j + r * I::B2);
+            stage_bfly<N, P::R2, NS2, DIR, T>(v, j, tw);
+        }
+        return j < I::B2;
+    } else {
+        constexpr int NS1 = P::R0;
+        if (j < I::B1) {
+#pragma unroll
+            for (int r = 0; r < P::R1; ++r) v[r] = *lds(j + r * I::B1);
+            stage_bfly<N, P::R1, NS1, DIR, T>(v, j, tw);
+        }
+        return j < I::B1;
+    }
+}
+
+// Full line transform; `store(p, v)` consumes output position p.
+template <int N, int DIR, typename T, class Load, class Lds, class Store>
+__device__ __forceinline__ void line_fft(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds, Store store) {
+    using I = PlanInfo<N>;
+    cpx<T> v[I::RMAX];
+    if (line_fft_regs<N, DIR, T>(j, tw, load, lds, v)) {
+#pragma unroll
+        for (int r = 0; r < I::RLAST; ++r) store(j + r * I::BLAST, v[r]);
+    }
+}
+
+// Stored row s of a line kept in compact form: cnt == N is the natural FFT
+// order; cnt = 2h+1 < N stores indices 0..h then N-h..N-1 (signed -h..-1).
+__device__ __forceinline__ int pos_to_stored(int p, int n, int cnt) {
+    if (cnt == n) return p;
+    const int h = (cnt - 1) >> 1;
+    if (p <= h) return p;
+    if (p >= n - h) return cnt - (n - p);
+    return -1;
+}
+
+}  // namespace laps

@@ -1,0 +1,246 @@
+// fft_kernels.hip -- batched 1D transforms of the 3D R2C/C2R pipeline.
+//
+// These replace the FFTW pencil loops and the MPI transposes of the
+// reference (fftw.f90:42-222, parallel.f90:218-324): on one GPU the data
+// never leaves HBM, and each pass reads its lines in whatever axis order the
+// layout gives, coalesced over a tile of adjacent lines.
+//
+//  k_strided  : C2C along y or z. A work-group owns TX adjacent lines
+//               (columns, stride 1 in memory) of one (batch1, batch2) slab;
+//               adjacent work-items touch adjacent columns, so each HBM
+//               access of a row of the tile is one 128-byte segment.
+//               Input and output rows may be stored pruned (kept ky/kz only).
+//  k_x_c2r    : x C2R of two rows at once (Z = A + iB, one complex line).
+//  k_x_r2c    : x R2C of two rows at once, split with the conjugate trick.
+#include "internal.hpp"
+
+namespace laps {
+
+template <typename T, int N>
+struct StridedCfg {
+    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
+    static constexpr int TPL = PlanInfo<N>::TPL;
+    static constexpr int NT = TX * TPL;
+    static constexpr int LS = N + 1;  // odd line stride: adjacent lines hit different banks
+};
+
+template <typename TI, typename T, int N, int DIR>
+__global__ __launch_bounds__((StridedCfg<T, N>::NT)) void k_strided(const cpx<TI>* __restrict__ in,
+                                                                  cpx<T>* __restrict__ out, StridedArgs a,
+                                                                  const cpx<T>* __restrict__ tw, T scale,
+                                                                  int ntile) {
+    using C = StridedCfg<T, N>;
+    __shared__ cpx<T> sh[C::TX * C::LS];
+    const int t = threadIdx.x;
+    const int l = t % C::TX, j = t / C::TX;
+    long long bid = blockIdx.x;
+    const int tile = (int)(bid % ntile);
+    bid /= ntile;
+    const int b1 = (int)(bid % a.nb1);
+    const int b2 = (int)(bid / a.nb1);
+    const int col = tile * C::TX + l;
+    const bool valid = col < a.ncol;
+    const cpx<TI>* ib = in + b1 * a.in_b1 + b2 * a.in_b2 + (valid ? col : 0);
+    cpx<T>* ob = out + b1 * a.out_b1 + b2 * a.out_b2 + (valid ? col : 0);
+    cpx<T>* line = sh + l * C::LS;
+    const int in_cnt = a.in_cnt, out_cnt = a.out_cnt;
+    const long long in_ls = a.in_ls, out_ls = a.out_ls;
+    auto load = [&](int p) -> cpx<T> {
+        const int s = pos_to_stored(p, N, in_cnt);
+        if (!valid || s < 0) return cpx<T>{T(0), T(0)};
+        const cpx<TI> v = ib[s * in_ls];
+        return cpx<T>{T(v.x), T(v.y)};
+    };
+    auto lds = [&](int p) { return line + p; };
+    auto store = [&](int p, cpx<T> v) {
+        const int s = pos_to_stored(p, N, out_cnt);
+        if (valid && s >= 0) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
+    };
+    line_fft<N, DIR, T>(j, tw, load, lds, store);
+}
+
+template <typename T, int N>
+struct XCfg {
+    static constexpr int TPL = PlanInfo<N>::TPL;
+    static constexpr int LP = TPL >= 128 ? 1 : 128 / TPL;  // line pairs per work-group
+    static constexpr int NT = LP * TPL;
+    static constexpr int PADSH = sizeof(T) == 8 ? 4 : 5;    // one pad slot per 256 bytes
+    static constexpr int LS = N + (N >> PADSH) + 1;
+    __device__ static constexpr int pad(int p) { return p + (p >> PADSH); }
+};
+
+// x C2R, unnormalised (plan_ifft_x, fftw.f90:31, :94-100). Rows 2q and 2q+1
+// are packed as Z_p = A_p + i B_p over the Hermitian extension; only the
+// real parts of the DC and Nyquist coefficients are read.
+template <typename T, int N>
+__global__ __launch_bounds__((XCfg<T, N>::NT)) void k_x_c2r(const cpx<T>* __restrict__ in, T* __restrict__ out,
+                                                          XArgs a, const cpx<T>* __restrict__ tw) {
+    using C = XCfg<T, N>;
+    __shared__ cpx<T> sh[C::LP * C::LS];
+    const int l = threadIdx.x / C::TPL, j = threadIdx.x % C::TPL;
+    const long long q = (long long)blockIdx.x * C::LP + l;
+    const bool valid = 2 * q + 1 < a.nrows;
+    const long long qq = valid ? q : 0;
+    const cpx<T>* ra = in + (2 * qq) * a.crow;
+    const cpx<T>* rb = ra + a.crow;
+    T* oa = out + (2 * qq) * a.rrow;
+    T* ob = oa + a.rrow;
+    const int kx = a.kx;
+    cpx<T>* line = sh + l * C::LS;
+    auto load = [&](int p) -> cpx<T> {
+        if (!valid) return cpx<T>{T(0), T(0)};
+        if (p == 0) return cpx<T>{ra[0].x, rb[0].x};
+        if (p < N / 2) {
+            if (p >= kx) return cpx<T>{T(0), T(0)};
+            const cpx<T> A = ra[p], B = rb[p];
+            return cpx<T>{A.x - B.y, A.y + B.x};
+        }
+        if (p == N / 2) {
+            if (p >= kx) return cpx<T>{T(0), T(0)};
+            return cpx<T>{ra[p].x, rb[p].x};
+        }
+        const int m = N - p;
+        if (m >= kx) return cpx<T>{T(0), T(0)};
+        const cpx<T> A = ra[m], B = rb[m];
+        return cpx<T>{A.x + B.y, B.x - A.y};
+    };
+    auto lds = [&](int p) { return line + C::pad(p); };
+    auto store = [&](int p, cpx<T> v) {
+        if (valid) {
+            oa[p] = v.x;
+            ob[p] = v.y;
+        }
+    };
+    line_fft<N, +1, T>(j, tw, load, lds, store);
+}
+
+// x R2C, unnormalised (plan_fft_x, fftw.f90:27): Z = FFT(a + i b);
+// A_k = (Z_k + conj Z_{N-k})/2, B_k = (Z_k - conj Z_{N-k})/(2i), k < kx.
+template <typename T, int N>
+__global__ __launch_bounds__((XCfg<T, N>::NT)) void k_x_r2c(const T* __restrict__ in, cpx<T>* __restrict__ out,
+                                                          XArgs a, const cpx<T>* __restrict__ tw) {
+    using C = XCfg<T, N>;
+    using I = PlanInfo<N>;
+    __shared__ cpx<T> sh[C::LP * C::LS];
+    const int l = threadIdx.x / C::TPL, j = threadIdx.x % C::TPL;
+    const long long q = (long long)blockIdx.x * C::LP + l;
+    const bool valid = 2 * q + 1 < a.nrows;
+    const long long qq = valid ? q : 0;
+    const T* ia = in + (2 * qq) * a.rrow;
+    const T* ib = ia + a.rrow;
+    cpx<T>* oa = out + (2 * qq) * a.crow;
+    cpx<T>* ob = oa + a.crow;
+    cpx<T>* line = sh + l * C::LS;
+    auto load = [&](int p) -> cpx<T> {
+        if (!valid) return cpx<T>{T(0), T(0)};
+        return cpx<T>{ia[p], ib[p]};
+    };
+    auto lds = [&](int p) { return line + C::pad(p); };
+    cpx<T> v[I::RMAX];
+    const bool has = line_fft_regs<N, -1, T>(j, tw, load, lds, v);
+    __syncthreads();
+    if (has) {
+#pragma unroll
+        for (int r = 0; r < I::RLAST; ++r) *lds(j + r * I::BLAST) = v[r];
+    }
+    __syncthreads();
+    if (valid) {
+        const T h = T(0.5);
+        for (int k = j; k < a.kx; k += C::TPL) {
+            const cpx<T> zk = *lds(k);
+            const cpx<T> zn = *lds((N - k) & (N - 1));
+            oa[k] = cpx<T>{h * (zk.x + zn.x), h * (zk.y - zn.y)};
+            ob[k] = cpx<T>{h * (zk.y + zn.y), h * (zn.x - zk.x)};
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <typename TI, typename T, int N, int DIR>
+static hipError_t do_strided(const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw, T scale,
+                             hipStream_t s) {
+    using C = StridedCfg<T, N>;
+    const int ntile = (a.ncol + C::TX - 1) / C::TX;
+    const long long nblk = (long long)ntile * a.nb1 * a.nb2;
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_strided<TI, T, N, DIR>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, scale,
+                       ntile);
+    return hipGetLastError();
+}
+
+template <typename TI, typename T>
+hipError_t launch_strided(int n, int dir, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw,
+                          T scale, hipStream_t s) {
+#define LAPS_STRIDED_CASE(NN)                                                              \
+    case NN:                                                                               \
+        return dir < 0 ? do_strided<TI, T, NN, -1>(in, out, a, tw, scale, s)               \
+                       : do_strided<TI, T, NN, +1>(in, out, a, tw, scale, s);
+    switch (n) {
+        LAPS_STRIDED_CASE(16)
+        LAPS_STRIDED_CASE(32)
+        LAPS_STRIDED_CASE(64)
+        LAPS_STRIDED_CASE(128)
+        LAPS_STRIDED_CASE(256)
+        LAPS_STRIDED_CASE(512)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef LAPS_STRIDED_CASE
+}
+
+template <typename T, int N>
+static hipError_t do_x(bool c2r, const void* in, void* out, const XArgs& a, const cpx<T>* tw, hipStream_t s) {
+    using C = XCfg<T, N>;
+    const long long npairs = a.nrows / 2;
+    const long long nblk = (npairs + C::LP - 1) / C::LP;
+    if (nblk <= 0) return hipSuccess;
+    if (c2r)
+        hipLaunchKernelGGL((k_x_c2r<T, N>), dim3((unsigned)nblk), dim3(C::NT), 0, s, (const cpx<T>*)in, (T*)out, a,
+                           tw);
+    else
+        hipLaunchKernelGGL((k_x_r2c<T, N>), dim3((unsigned)nblk), dim3(C::NT), 0, s, (const T*)in, (cpx<T>*)out, a,
+                           tw);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t dispatch_x(int n, bool c2r, const void* in, void* out, const XArgs& a, const cpx<T>* tw,
+                             hipStream_t s) {
+    if (a.nrows % 2) return hipErrorInvalidValue;
+    switch (n) {
+        case 16: return do_x<T, 16>(c2r, in, out, a, tw, s);
+        case 32: return do_x<T, 32>(c2r, in, out, a, tw, s);
+        case 64: return do_x<T, 64>(c2r, in, out, a, tw, s);
+        case 128: return do_x<T, 128>(c2r, in, out, a, tw, s);
+        case 256: return do_x<T, 256>(c2r, in, out, a, tw, s);
+        case 512: return do_x<T, 512>(c2r, in, out, a, tw, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_x_c2r(int n, const cpx<T>* in, T* out, const XArgs& a, const cpx<T>* tw, hipStream_t s) {
+    return dispatch_x<T>(n, true, in, out, a, tw, s);
+}
+template <typename T>
+hipError_t launch_x_r2c(int n, const T* in, cpx<T>* out, const XArgs& a, const cpx<T>* tw, hipStream_t s) {
+    return dispatch_x<T>(n, false, in, out, a, tw, s);
+}
+
+template hipError_t launch_strided<double, double>(int, int, const cpx<double>*, cpx<double>*, const StridedArgs&,
+                                                   const cpx<double>*, double, hipStream_t);
+template hipError_t launch_strided<double, float>(int, int, const cpx<double>*, cpx<float>*, const StridedArgs&,
+                                                  const cpx<float>*, float, hipStream_t);
+template hipError_t launch_strided<float, float>(int, int, const cpx<float>*, cpx<float>*, const StridedArgs&,
+                                                 const cpx<float>*, float, hipStream_t);
+template hipError_t launch_x_c2r<double>(int, const cpx<double>*, double*, const XArgs&, const cpx<double>*,
+                                         hipStream_t);
+template hipError_t launch_x_c2r<float>(int, const cpx<float>*, float*, const XArgs&, const cpx<float>*,
+                                        hipStream_t);
+template hipError_t launch_x_r2c<double>(int, const double*, cpx<double>*, const XArgs&, const cpx<double>*,
+                                         hipStream_t);
+template hipError_t launch_x_r2c<float>(int, const float*, cpx<float>*, const XArgs&, const cpx<float>*,
+                                        hipStream_t);
+
+}  // namespace laps

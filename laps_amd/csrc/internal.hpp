@@ -1,0 +1,111 @@
+// internal.hpp -- host-side declarations shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "fft_dev.hpp"
+
+namespace laps {
+
+// Strided batched C2C along one axis. Lines run along the axis with element
+// stride `ls`; `ncol` adjacent lines (stride 1) form a column tile; two batch
+// dimensions select the tile. Stored positions may be pruned (cnt < N).
+struct StridedArgs {
+    int ncol;
+    int in_cnt, out_cnt;
+    long long in_ls, out_ls;
+    int nb1;
+    long long in_b1, out_b1;
+    int nb2;
+    long long in_b2, out_b2;
+};
+
+// Rows of an x transform: `nrows` rows (even), complex rows hold `kx` stored
+// coefficients with stride `crow`, real rows hold N values with stride `rrow`.
+struct XArgs {
+    long long nrows;
+    int kx;
+    long long crow, rrow;
+};
+
+// Device control block of one box (scalars of mhd.f90 / AEBmod.f90).
+struct Ctl {
+    double time, dt, dt_prev, radius, tau, r0r;
+    double cc[3], dd[3], ts[3];
+    unsigned long long dtmin_bits;
+    long long istep;
+    int status;
+    int ksq_updated;
+};
+
+// Kept-mode box of the spectral state: kx in [0,KX), ky/kz stored compactly.
+struct Box {
+    int KX, CY, CZ;
+    __host__ __device__ long long modes() const { return (long long)KX * CY * CZ; }
+};
+
+// Per-box wave-number and dealias tables (device pointers).
+struct BoxTables {
+    const double* wx;      // [KX]
+    const double* wy;      // [CY]
+    const double* wz;      // [CZ]
+    const int* kzlim;      // [CY*KX], option 1: keep |mz| <= kzlim (-1: none)
+    const double* fx;      // [KX] option 2 filters
+    const double* fy;      // [CY]
+    const double* fz;      // [CZ]
+    const int* mz;         // [CZ] signed integer index of stored z row
+};
+
+struct PhysParams {
+    double gamma, di, nu, eta, radius0, Ur, cos_c, sin_c, cfl, fixed_dt;
+    double dx, dy, dz;
+    int hall, aeb, corot_k, corot_ksq, visc, visc_exp, resis, resis_exp, conserve_bg, dealias;
+};
+
+template <typename T>
+struct Twiddles {
+    const cpx<T>* x;
+    const cpx<T>* y;
+    const cpx<T>* z;
+};
+
+// --- FFT launchers (fft_kernels.hip) ---
+template <typename TI, typename T>
+hipError_t launch_strided(int n, int dir, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a,
+                          const cpx<T>* tw, T scale, hipStream_t s);
+template <typename T>
+hipError_t launch_x_c2r(int n, const cpx<T>* in, T* out, const XArgs& a, const cpx<T>* tw, hipStream_t s);
+template <typename T>
+hipError_t launch_x_r2c(int n, const T* in, cpx<T>* out, const XArgs& a, const cpx<T>* tw, hipStream_t s);
+
+// --- physics launchers (physics_kernels.hip) ---
+hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fstride, Box b, BoxTables t,
+                       const PhysParams& pp, const Ctl* ctl, hipStream_t s);
+template <typename T>
+hipError_t launch_flux(const T* R, const double* IC, T* F, long long npts, int nflux, bool first, bool cfl,
+                       const PhysParams& pp, Ctl* ctl, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs_rk(const cpx<T>* E, cpx<double>* U, cpx<double>* Fp, Box b, BoxTables t, int irk,
+                         const PhysParams& pp, const Ctl* ctl, hipStream_t s);
+hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s);
+hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s);
+hipError_t launch_ctl_end(Ctl* ctl, const PhysParams& pp, hipStream_t s);
+template <typename T>
+hipError_t launch_prim(const T* R, double* P, long long npts, double gamma, hipStream_t s);
+hipError_t launch_cfl(const double* U8, const double* P4, long long npts, const PhysParams& pp, Ctl* ctl,
+                      hipStream_t s);
+hipError_t launch_moments(const double* U8, const double* P4, long long npts, double* partial, int nblk,
+                          hipStream_t s);
+hipError_t launch_moments2(const double* U8, const double* P4, long long npts, const double* ave, double* partial,
+                           int nblk, hipStream_t s);
+hipError_t launch_divb(const cpx<double>* U, long long fstride, Box b, BoxTables t, const PhysParams& pp,
+                       const Ctl* ctl, unsigned long long* out, hipStream_t s);
+hipError_t launch_gather_box(const cpx<double>* full, cpx<double>* box, int nf, long long full_stride,
+                             long long box_stride, int nxh, int ny, int nz, Box b, hipStream_t s);
+template <typename TI, typename TO>
+hipError_t launch_convert(const TI* in, TO* out, long long n, hipStream_t s);
+
+}  // namespace laps

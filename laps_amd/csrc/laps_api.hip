@@ -1,0 +1,894 @@
+// laps_api.hip -- the C ABI of include/laps.h and the orchestration of one
+// RK3 step on the device.
+//
+// Device data layout (one box):
+//   U      [11][CZ][CY][KX] FP64 complex : spectral state uu_fourier (8) +
+//                                         Hall current J_hat (3), kept box
+//   Ufull  [11][nz][ny][nx/2+1] FP64     : un-dealiased spectrum of the state
+//                                         handed over by laps_set_state
+//   Fp     [8][CZ][CY][KX] FP64 complex  : previous RHS fnl_rk
+//   S      [12][nz][ny][nx] FP64 real    : real-space state uu(8)+uu_prim(4)
+//                                         (the uploaded one, or materialised)
+//   P1,P2  T complex, R,Fr T real        : transform intermediates, T = FP64
+//                                         or FP32 (mixed precision)
+// The "kept box" holds kx < KX and the stored ky/kz rows of the modes that
+// survive dealiasing: all modes for the compact filter (option 2), the
+// bounding box of the 2/3 sphere for option 1. Modes outside the box are
+// identically zero after every substage (dealiasing.f90:152-164), so the
+// transforms read and write only the box.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/laps.h"
+#include "internal.hpp"
+
+using namespace laps;
+
+static_assert(sizeof(laps_params) == 168, "laps_params layout is part of the ABI");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define LAPS_CK(...)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (__VA_ARGS__);                                                             \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(LAPS_EDEVICE, std::string(#__VA_ARGS__) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+constexpr double kPiRef = 3.141592653589793;  // mhdinit.f90:7
+constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+
+// exp(-2 pi i t / n), angle reduced to one octant in long double
+void root(long t, long n, double& c, double& s) {
+    t %= n;
+    const long q8 = 8 * t, oct = q8 / n, rem = q8 - oct * n;
+    const long double a = (oct % 2 == 0) ? kPiL / 4.0L * (long double)rem / (long double)n
+                                         : kPiL / 4.0L * (long double)(n - rem) / (long double)n;
+    const long double ca = cosl(a), sa = sinl(a);
+    long double cr, sr;
+    switch (oct) {
+        case 0: cr = ca; sr = sa; break;
+        case 1: cr = sa; sr = ca; break;
+        case 2: cr = -sa; sr = ca; break;
+        case 3: cr = -ca; sr = sa; break;
+        case 4: cr = -ca; sr = -sa; break;
+        case 5: cr = -sa; sr = -ca; break;
+        case 6: cr = sa; sr = -ca; break;
+        default: cr = ca; sr = -sa; break;
+    }
+    c = (double)cr;
+    s = -(double)sr;
+}
+
+// wave_number_* (mhdinit.f90:83-111): 2*pi*m/L, m signed, Nyquist carries +n/2
+double wave_number(int i, int n, double L) {
+    const int m = (i <= n / 2) ? i : i - n;
+    return 2 * kPiRef * (double)m / L;
+}
+
+int signed_index(int i, int n) { return (i <= n / 2) ? i : i - n; }
+
+// dealiasing.f90:101-108
+double compact_filter(double w, double L, int n, double af) {
+    const double aj = (5.0 + 6.0 * af) / 8.0;
+    const double bj = (1.0 + 2.0 * af) / 2.0;
+    const double cj = -(1.0 - 2 * af) / 8.0;
+    const double wl = w * L / n;
+    return (aj + bj * cos(wl) + cj * cos(2 * wl)) / (1 + 2 * af * cos(wl));
+}
+
+bool supported_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+
+struct DevTables {
+    double *wx = nullptr, *wy = nullptr, *wz = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    int *kzlim = nullptr, *mz = nullptr;
+    BoxTables view() const { return BoxTables{wx, wy, wz, kzlim, fx, fy, fz, mz}; }
+};
+
+}  // namespace
+
+// Kernel classes of the step pipeline, timed with HIP events on the
+// context's stream when laps_timing_enable() is on.
+enum KClass { K_ZINV, K_YINV, K_XC2R, K_FLUX, K_XR2C, K_YFWD, K_ZFWD, K_RHSRK, K_CURL, K_CTL, K_NCLASS };
+static const char* const kClassNames[K_NCLASS] = {
+    "fft_z_inverse", "fft_y_inverse", "fft_x_c2r", "flux_cfl", "fft_x_r2c",
+    "fft_y_forward", "fft_z_forward", "rhs_rk3_dealias", "hall_curl", "control"};
+
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    double ms[K_NCLASS] = {0};
+    double bytes[K_NCLASS] = {0};
+    long long launches[K_NCLASS] = {0};
+};
+
+struct laps_ctx {
+    Timing timing;
+    laps_params p{};
+    int nx = 0, ny = 0, nz = 0, nxh = 0;
+    long long N = 0, M = 0;
+    bool mixed = false;
+    hipStream_t stream = nullptr;
+    Box full{}, kept{};
+    DevTables tfull, tkept;
+    PhysParams pp{};
+    Ctl* ctl = nullptr;
+    cpx<double>* twd[3] = {nullptr, nullptr, nullptr};
+    cpx<float>* twf[3] = {nullptr, nullptr, nullptr};
+    cpx<double>* U = nullptr;
+    cpx<double>* Ufull = nullptr;
+    cpx<double>* Fp = nullptr;
+    double* S = nullptr;
+    void* P1 = nullptr;
+    void* P2 = nullptr;
+    void* R = nullptr;
+    void* Fr = nullptr;
+    cpx<double>* X1 = nullptr;
+    cpx<double>* X2 = nullptr;
+    double* Xr = nullptr;
+    double* red = nullptr;
+    double* dave = nullptr;
+    unsigned long long* dbits = nullptr;
+    std::vector<void*> allocs;
+    bool have_state = false, fresh = false, materialized = false, stepped_since_set = false;
+    double fixed_dt = 0.0;
+    int nblk_red = 1024;
+
+    int alloc(void** ptr, size_t bytes) {
+        if (bytes == 0) bytes = 16;
+        hipError_t e = hipMalloc(ptr, bytes);
+        if (e != hipSuccess) {
+            *ptr = nullptr;
+            return fail(LAPS_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+        }
+        allocs.push_back(*ptr);
+        return LAPS_OK;
+    }
+    template <typename P>
+    int allocn(P** ptr, size_t count) {
+        return alloc((void**)ptr, count * sizeof(P));
+    }
+    long long kb() const { return kept.modes(); }
+    int nf_in() const { return p.if_hall ? 11 : 8; }
+    int nf_out() const { return p.if_aeb ? 19 : 18; }
+};
+
+namespace {
+
+// ------------------------------------------------------------------ setup helpers
+
+int upload_tables(laps_ctx* c, const Box& b, bool pruned, const std::vector<int>& kzlim, DevTables& t) {
+    std::vector<double> wx(b.KX), fx(b.KX), wy(b.CY), fy(b.CY), wz(b.CZ), fz(b.CZ);
+    std::vector<int> mz(b.CZ);
+    auto fft_index = [](int s, int cnt, int n) { return (cnt == n || s <= (cnt - 1) / 2) ? s : n - (cnt - s); };
+    for (int i = 0; i < b.KX; ++i) {
+        wx[i] = wave_number(i, c->nx, c->p.Lx);
+        fx[i] = compact_filter(wx[i], c->p.Lx, c->nx, c->p.afx);
+    }
+    for (int s = 0; s < b.CY; ++s) {
+        const int iy = fft_index(s, b.CY, c->ny);
+        wy[s] = wave_number(iy, c->ny, c->p.Ly);
+        fy[s] = compact_filter(wy[s], c->p.Ly, c->ny, c->p.afy);
+    }
+    for (int s = 0; s < b.CZ; ++s) {
+        const int iz = fft_index(s, b.CZ, c->nz);
+        wz[s] = wave_number(iz, c->nz, c->p.Lz);
+        fz[s] = compact_filter(wz[s], c->p.Lz, c->nz, c->p.afz);
+        mz[s] = signed_index(iz, c->nz);
+    }
+    std::vector<int> lim = kzlim;
+    if (!pruned) lim.assign((size_t)b.KX * b.CY, c->nz);
+    int rc;
+    if ((rc = c->allocn(&t.wx, b.KX)) || (rc = c->allocn(&t.fx, b.KX)) || (rc = c->allocn(&t.wy, b.CY)) ||
+        (rc = c->allocn(&t.fy, b.CY)) || (rc = c->allocn(&t.wz, b.CZ)) || (rc = c->allocn(&t.fz, b.CZ)) ||
+        (rc = c->allocn(&t.mz, b.CZ)) || (rc = c->allocn(&t.kzlim, lim.size())))
+        return rc;
+    LAPS_CK(hipMemcpy(t.wx, wx.data(), wx.size() * 8, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.fx, fx.data(), fx.size() * 8, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.wy, wy.data(), wy.size() * 8, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.fy, fy.data(), fy.size() * 8, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.wz, wz.data(), wz.size() * 8, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.fz, fz.data(), fz.size() * 8, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.mz, mz.data(), mz.size() * 4, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.kzlim, lim.data(), lim.size() * 4, hipMemcpyHostToDevice));
+    return LAPS_OK;
+}
+
+// Bounding box of the modes kept by the 2/3 rule (dealiasing.f90:152-164),
+// evaluated with the reference's expression, plus the per-column |mz| limit.
+int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
+    const int nx = c->nx, ny = c->ny, nz = c->nz, nxh = c->nxh;
+    if (c->p.dealias_option != 1) {
+        c->kept = c->full;
+        return LAPS_OK;
+    }
+    std::vector<double> ax(nxh), ay(ny), az(nz);
+    for (int i = 0; i < nxh; ++i) { const double w = wave_number(i, nx, c->p.Lx); ax[i] = w * c->p.Lx / (2 * kPiRef * nx); }
+    for (int i = 0; i < ny; ++i) { const double w = wave_number(i, ny, c->p.Ly); ay[i] = w * c->p.Ly / (2 * kPiRef * ny); }
+    for (int i = 0; i < nz; ++i) { const double w = wave_number(i, nz, c->p.Lz); az[i] = w * c->p.Lz / (2 * kPiRef * nz); }
+    auto keep = [&](int ix, int iy, int iz) {
+        const double r = sqrt(ax[ix] * ax[ix] + ay[iy] * ay[iy] + az[iz] * az[iz]);
+        return !(r > 1. / 3.);
+    };
+    int kxmax = -1, kymax = 0, kzmax = 0;
+    std::vector<int> col((size_t)nxh * ny, -1);
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nxh; ++ix) {
+            int lim = -1;
+            for (int iz = 0; iz < nz; ++iz)
+                if (keep(ix, iy, iz)) lim = std::max(lim, std::abs(signed_index(iz, nz)));
+            // the kept set of a column must be |mz| <= lim exactly
+            for (int iz = 0; iz < nz; ++iz) {
+                const bool inside = std::abs(signed_index(iz, nz)) <= lim;
+                if (inside != keep(ix, iy, iz)) return fail(LAPS_EINVAL, "2/3 mask is not a z-interval");
+            }
+            col[(size_t)iy * nxh + ix] = lim;
+            if (lim >= 0) {
+                kxmax = std::max(kxmax, ix);
+                kymax = std::max(kymax, std::abs(signed_index(iy, ny)));
+                kzmax = std::max(kzmax, lim);
+            }
+        }
+    if (kxmax < 0) return fail(LAPS_EINVAL, "2/3 mask keeps no mode");
+    Box b;
+    b.KX = kxmax + 1;
+    b.CY = std::min(2 * kymax + 1, ny);
+    b.CZ = std::min(2 * kzmax + 1, nz);
+    if (b.CY == ny - 1) b.CY = ny;  // keep compact rows symmetric or full
+    if (b.CZ == nz - 1) b.CZ = nz;
+    c->kept = b;
+    kzlim.assign((size_t)b.KX * b.CY, -1);
+    for (int cy = 0; cy < b.CY; ++cy) {
+        const int iy = (b.CY == ny || cy <= (b.CY - 1) / 2) ? cy : ny - (b.CY - cy);
+        for (int kx = 0; kx < b.KX; ++kx) kzlim[(size_t)cy * b.KX + kx] = col[(size_t)iy * nxh + kx];
+    }
+    return LAPS_OK;
+}
+
+// ------------------------------------------------------------------ kernel timing
+
+hipEvent_t take_event(laps_ctx* c) {
+    if (!c->timing.pool.empty()) {
+        hipEvent_t e = c->timing.pool.back();
+        c->timing.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs `launch` (returning hipError_t); with timing on, brackets it with
+// events on the context stream and books `bytes` of algorithmic traffic.
+template <class F>
+hipError_t timed(laps_ctx* c, int cls, double bytes, F launch) {
+    if (!c->timing.on) return launch();
+    hipEvent_t a = take_event(c), b = take_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+    hipError_t e = launch();
+    if (b) (void)hipEventRecord(b, c->stream);
+    if (a && b) c->timing.pending.push_back({cls, {a, b}});
+    c->timing.bytes[cls] += bytes;
+    c->timing.launches[cls] += 1;
+    return e;
+}
+
+int collect_timing(laps_ctx* c) {
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    for (auto& pe : c->timing.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) c->timing.ms[pe.first] += ms;
+        c->timing.pool.push_back(pe.second.first);
+        c->timing.pool.push_back(pe.second.second);
+    }
+    c->timing.pending.clear();
+    return LAPS_OK;
+}
+
+// ------------------------------------------------------------------ pipelines
+
+template <typename T>
+const cpx<T>* tw_of(laps_ctx* c, int axis);
+template <>
+const cpx<double>* tw_of<double>(laps_ctx* c, int axis) { return c->twd[axis]; }
+template <>
+const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
+
+// spectral box (fields with stride fs) -> real fields, unnormalised
+// (transform_uu_fourier_to_real, fftw.f90:73-103)
+template <typename T>
+int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
+                 cpx<T>* B) {
+    const long long NX = c->nx, NY = c->ny, NZ = c->nz;
+    StridedArgs z{};
+    z.ncol = b.KX; z.in_cnt = b.CZ; z.out_cnt = (int)NZ;
+    z.in_ls = (long long)b.CY * b.KX; z.out_ls = (long long)b.CY * b.KX;
+    z.nb1 = b.CY; z.in_b1 = b.KX; z.out_b1 = b.KX;
+    z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * b.KX;
+    const double ez = (double)nf * b.CY * b.KX;
+    LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + NZ * 2.0 * sizeof(T)),
+                  [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream); }));
+    StridedArgs y{};
+    y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
+    y.in_ls = b.KX; y.out_ls = b.KX;
+    y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * b.KX; y.out_b1 = NY * b.KX;
+    y.nb2 = nf; y.in_b2 = NZ * b.CY * b.KX; y.out_b2 = NZ * NY * b.KX;
+    const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
+    LAPS_CK(timed(c, K_YINV, ey * (b.CY + NY),
+                  [&] { return launch_strided<T, T>(c->ny, +1, A, B, y, tw_of<T>(c, 1), T(1), c->stream); }));
+    XArgs x{};
+    x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = b.KX; x.rrow = NX;
+    LAPS_CK(timed(c, K_XC2R, (double)x.nrows * sizeof(T) * (2.0 * b.KX + NX),
+                  [&] { return launch_x_c2r<T>(c->nx, B, dst, x, tw_of<T>(c, 0), c->stream); }));
+    return LAPS_OK;
+}
+
+// real fields -> spectral box in A, normalised by 1/(nx ny nz)
+// (transform_flux_real_to_fourier, mhdrhs.f90:128-172; fftw.f90:42-71)
+template <typename T>
+int fwd_pipeline(laps_ctx* c, const T* src, int nf, const Box& b, cpx<T>* A, cpx<T>* B) {
+    const long long NX = c->nx, NY = c->ny, NZ = c->nz;
+    XArgs x{};
+    x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = b.KX; x.rrow = NX;
+    LAPS_CK(timed(c, K_XR2C, (double)x.nrows * sizeof(T) * (2.0 * b.KX + NX),
+                  [&] { return launch_x_r2c<T>(c->nx, src, A, x, tw_of<T>(c, 0), c->stream); }));
+    StridedArgs y{};
+    y.ncol = b.KX; y.in_cnt = (int)NY; y.out_cnt = b.CY;
+    y.in_ls = b.KX; y.out_ls = b.KX;
+    y.nb1 = (int)NZ; y.in_b1 = NY * b.KX; y.out_b1 = (long long)b.CY * b.KX;
+    y.nb2 = nf; y.in_b2 = NZ * NY * b.KX; y.out_b2 = NZ * b.CY * b.KX;
+    const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
+    LAPS_CK(timed(c, K_YFWD, ey * (NY + b.CY),
+                  [&] { return launch_strided<T, T>(c->ny, -1, A, B, y, tw_of<T>(c, 1), T(1), c->stream); }));
+    StridedArgs z{};
+    z.ncol = b.KX; z.in_cnt = (int)NZ; z.out_cnt = b.CZ;
+    z.in_ls = (long long)b.CY * b.KX; z.out_ls = (long long)b.CY * b.KX;
+    z.nb1 = b.CY; z.in_b1 = b.KX; z.out_b1 = b.KX;
+    z.nb2 = nf; z.in_b2 = NZ * b.CY * b.KX; z.out_b2 = (long long)b.CZ * b.CY * b.KX;
+    const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
+    const double ez = (double)nf * b.CY * b.KX * 2.0 * sizeof(T);
+    LAPS_CK(timed(c, K_ZFWD, ez * (NZ + b.CZ),
+                  [&] { return launch_strided<T, T>(c->nz, -1, B, A, z, tw_of<T>(c, 2), scale, c->stream); }));
+    return LAPS_OK;
+}
+
+// one substage of evolve (mhd.f90:303-325)
+template <typename T>
+int substage(laps_ctx* c, int irk) {
+    const bool first = c->fresh && irk == 0;
+    const bool hall = c->p.if_hall != 0;
+    const long long KB = c->kb();
+    T* R = (T*)c->R;
+    T* Fr = (T*)c->Fr;
+    cpx<T>* P1 = (cpx<T>*)c->P1;
+    cpx<T>* P2 = (cpx<T>*)c->P2;
+    int rc;
+    if (first) {
+        // first substage: the uploaded real fields are used as they are; J
+        // comes from the un-dealiased spectrum (mhd.f90:121-122, :305)
+        if (hall) {
+            LAPS_CK(timed(c, K_CURL, 96.0 * c->M, [&] {
+                return launch_curl(c->Ufull, c->Ufull + 8 * c->M, c->M, c->full, c->tfull.view(), c->pp, c->ctl,
+                                   c->stream);
+            }));
+            if ((rc = inv_pipeline<T>(c, c->Ufull + 8 * c->M, c->M, c->full, 3, R + 8 * c->N, P1, P2))) return rc;
+        }
+    } else {
+        if (hall)
+            LAPS_CK(timed(c, K_CURL, 96.0 * KB, [&] {
+                return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
+            }));
+        if ((rc = inv_pipeline<T>(c, c->U, KB, c->kept, c->nf_in(), R, P1, P2))) return rc;
+    }
+    const bool cfl = irk == 0 && !(c->fixed_dt > 0);
+    if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_begin(c->ctl, c->stream); }));
+    const double fin = first ? (12.0 * 8 + (hall ? 3.0 * sizeof(T) : 0.0)) : c->nf_in() * (double)sizeof(T);
+    LAPS_CK(timed(c, K_FLUX, (double)c->N * (fin + c->nf_out() * sizeof(T)), [&] {
+        return launch_flux<T>(R, c->S, Fr, c->N, c->nf_out(), first, cfl, c->pp, c->ctl, c->stream);
+    }));
+    if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_finalize(c->ctl, c->pp, c->stream); }));
+    if ((rc = fwd_pipeline<T>(c, Fr, c->nf_out(), c->kept, P1, P2))) return rc;
+    const double rb = (double)KB * (c->nf_out() * 2.0 * sizeof(T) + 16.0 * (8 + 8 + 8 + (irk ? 8 : 0)));
+    LAPS_CK(timed(c, K_RHSRK, rb, [&] {
+        return launch_rhs_rk<T>(P1, c->U, c->Fp, c->kept, c->tkept.view(), irk, c->pp, c->ctl, c->stream);
+    }));
+    return LAPS_OK;
+}
+
+template <typename T>
+int one_step(laps_ctx* c) {
+    int rc;
+    for (int irk = 0; irk < 3; ++irk)
+        if ((rc = substage<T>(c, irk))) return rc;
+    LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_end(c->ctl, c->pp, c->stream); }));
+    c->fresh = false;
+    c->materialized = false;
+    return LAPS_OK;
+}
+
+// FP64 single-field utilities (set_state, materialisation, laps_fft_*)
+struct TimingOff {
+    laps_ctx* c;
+    bool was;
+    explicit TimingOff(laps_ctx* cc) : c(cc), was(cc->timing.on) { c->timing.on = false; }
+    ~TimingOff() { c->timing.on = was; }
+};
+
+int fwd_fp64(laps_ctx* c, const double* real, cpx<double>* out_full) {
+    TimingOff off(c);
+    int rc = fwd_pipeline<double>(c, real, 1, c->full, c->X1, c->X2);
+    if (rc) return rc;
+    if (out_full != c->X1)
+        LAPS_CK(hipMemcpyAsync(out_full, c->X1, c->M * sizeof(cpx<double>), hipMemcpyDeviceToDevice, c->stream));
+    return LAPS_OK;
+}
+
+int inv_fp64(laps_ctx* c, const cpx<double>* src, const Box& b, double* real) {
+    TimingOff off(c);
+    return inv_pipeline<double>(c, src, b.modes(), b, 1, real, c->X1, c->X2);
+}
+
+// A step refused by the dt < 1e-8 test (status LAPS_DT_STOP) leaves the
+// state untouched; if no step was ever taken since laps_set_state, the
+// uploaded arrays are still the state (mhd.f90:216-242 writes them as is).
+int refresh_fresh(laps_ctx* c) {
+    if (c->fresh || !c->stepped_since_set) return LAPS_OK;
+    Ctl h;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (h.status != 0 && h.istep == 0) {
+        c->fresh = true;
+        c->materialized = true;
+    }
+    return LAPS_OK;
+}
+
+int materialize(laps_ctx* c) {
+    int rc0 = refresh_fresh(c);
+    if (rc0) return rc0;
+    if (c->materialized) return LAPS_OK;
+    int rc;
+    for (int f = 0; f < 8; ++f)
+        if ((rc = inv_fp64(c, c->U + f * c->kb(), c->kept, c->S + f * c->N))) return rc;
+    LAPS_CK(launch_prim<double>(c->S, c->S + 8 * c->N, c->N, c->p.adiabatic_index, c->stream));
+    c->materialized = true;
+    return LAPS_OK;
+}
+
+PhysParams make_phys(const laps_params& p) {
+    PhysParams pp{};
+    pp.gamma = p.adiabatic_index;
+    pp.di = p.ion_inertial_length;
+    pp.nu = p.viscosity;
+    pp.eta = p.resistivity;
+    pp.radius0 = p.radius0;
+    pp.Ur = p.if_aeb ? p.Ur0 : 0.0;  // mhd.f90:88-90
+    const double ang = p.if_corotating ? p.corotating_angle : 0.0;  // AEBmod.f90:28-30
+    pp.cos_c = cos(ang);
+    pp.sin_c = sin(ang);
+    pp.cfl = p.cfl;
+    pp.fixed_dt = 0.0;
+    pp.dx = p.Lx / p.nx;
+    pp.dy = p.Ly / p.ny;
+    pp.dz = p.Lz / p.nz;
+    pp.hall = p.if_hall != 0;
+    pp.aeb = p.if_aeb != 0;
+    pp.corot_k = p.if_aeb && p.if_corotating;
+    pp.corot_ksq = p.if_corotating != 0;
+    pp.visc = p.if_visc != 0;
+    pp.visc_exp = p.if_visc_exp != 0;
+    pp.resis = p.if_resis != 0;
+    pp.resis_exp = p.if_resis_exp != 0;
+    pp.conserve_bg = p.if_conserve_background != 0;
+    pp.dealias = p.dealias_option;
+    return pp;
+}
+
+int reset_ctl(laps_ctx* c, double time, double dt_prev, bool ksq_updated) {
+    Ctl h{};
+    h.time = time;
+    h.dt = 0.0;
+    h.dt_prev = dt_prev;
+    h.radius = c->pp.radius0 + c->pp.Ur * time;
+    h.tau = h.radius / c->pp.Ur;
+    h.r0r = c->pp.radius0 / h.radius;
+    h.dtmin_bits = 0x7FF0000000000000ULL;
+    h.istep = 0;
+    h.status = 0;
+    h.ksq_updated = ksq_updated ? 1 : 0;
+    LAPS_CK(hipMemcpy(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice));
+    return LAPS_OK;
+}
+
+int read_ctl(laps_ctx* c, Ctl& h) {
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    return LAPS_OK;
+}
+
+int check_ctx(laps_ctx* c, bool need_state) {
+    if (!c) return fail(LAPS_EINVAL, "null context");
+    if (need_state && !c->have_state) return fail(LAPS_ESTATE, "no state: call laps_set_state first");
+    hipError_t e = hipSetDevice(c->p.device);
+    if (e != hipSuccess) return fail(LAPS_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return LAPS_OK;
+}
+
+void free_all(laps_ctx* c) {
+    for (auto& pe : c->timing.pending) {
+        c->timing.pool.push_back(pe.second.first);
+        c->timing.pool.push_back(pe.second.second);
+    }
+    c->timing.pending.clear();
+    for (hipEvent_t e : c->timing.pool) (void)hipEventDestroy(e);
+    c->timing.pool.clear();
+    for (void* ptr : c->allocs) (void)hipFree(ptr);
+    c->allocs.clear();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+void laps_default_params(laps_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nx = 128; p->ny = 128; p->nz = 64;
+    p->precision = LAPS_FP64;
+    p->Lx = p->Ly = p->Lz = 2.0 * kPiRef;
+    p->adiabatic_index = 5. / 3.;
+    p->cfl = 0.5;
+    p->dealias_option = 2;
+    p->afx = p->afy = p->afz = 0.495;
+    p->radius0 = 30.0;
+}
+
+const char* laps_last_error(void) { return g_err.c_str(); }
+
+int laps_create(const laps_params* p, laps_ctx** out) {
+    if (!p || !out) return fail(LAPS_EINVAL, "null argument");
+    *out = nullptr;
+    if (!supported_len(p->nx) || !supported_len(p->ny) || !supported_len(p->nz))
+        return fail(LAPS_EINVAL, "grid lengths must be 16, 32, 64, 128, 256 or 512 per axis");
+    if (p->precision != LAPS_FP64 && p->precision != LAPS_MIXED) return fail(LAPS_EINVAL, "unknown precision");
+    if (p->dealias_option < 0 || p->dealias_option > 2) return fail(LAPS_EINVAL, "dealias_option must be 0, 1 or 2");
+    if (!(p->Lx > 0 && p->Ly > 0 && p->Lz > 0)) return fail(LAPS_EINVAL, "box lengths must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0)
+        return fail(LAPS_EDEVICE, "no HIP device " + std::to_string(p->device));
+    LAPS_CK(hipSetDevice(p->device));
+    laps_ctx* c = new laps_ctx();
+    c->p = *p;
+    c->nx = p->nx; c->ny = p->ny; c->nz = p->nz; c->nxh = p->nx / 2 + 1;
+    c->N = (long long)c->nx * c->ny * c->nz;
+    c->M = (long long)c->nxh * c->ny * c->nz;
+    c->mixed = p->precision == LAPS_MIXED;
+    c->full = Box{c->nxh, c->ny, c->nz};
+    c->pp = make_phys(*p);
+    int rc;
+    auto bail = [&](int code) {
+        free_all(c);
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(LAPS_EDEVICE, "hipStreamCreate failed"));
+    std::vector<int> kzlim;
+    if ((rc = compute_kept_box(c, kzlim))) return bail(rc);
+    if ((rc = upload_tables(c, c->full, false, {}, c->tfull))) return bail(rc);
+    if ((rc = upload_tables(c, c->kept, p->dealias_option == 1, kzlim, c->tkept))) return bail(rc);
+    // twiddle tables W_n^t for the three axis lengths, FP64 and FP32
+    const int lens[3] = {c->nx, c->ny, c->nz};
+    for (int a = 0; a < 3; ++a) {
+        const int n = lens[a];
+        std::vector<cpx<double>> d(n);
+        std::vector<cpx<float>> f(n);
+        for (int t = 0; t < n; ++t) {
+            root(t, n, d[t].x, d[t].y);
+            f[t].x = (float)d[t].x;
+            f[t].y = (float)d[t].y;
+        }
+        if ((rc = c->allocn(&c->twd[a], n)) || (rc = c->allocn(&c->twf[a], n))) return bail(rc);
+        if (hipMemcpy(c->twd[a], d.data(), n * sizeof(cpx<double>), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->twf[a], f.data(), n * sizeof(cpx<float>), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(LAPS_EDEVICE, "twiddle upload failed"));
+    }
+    const long long KB = c->kb();
+    const long long NX = c->nx, NY = c->ny, NZ = c->nz;
+    const Box& k = c->kept;
+    const int nfi = c->nf_in(), nfo = c->nf_out();
+    long long p1 = std::max({(long long)nfi * NZ * k.CY * k.KX, (long long)nfo * NZ * NY * k.KX,
+                             (long long)nfo * k.CZ * k.CY * k.KX, 3LL * NZ * NY * c->nxh});
+    long long p2 = std::max({(long long)nfi * NZ * NY * k.KX, (long long)nfo * NZ * k.CY * k.KX, 3LL * NZ * NY * c->nxh});
+    const size_t ts = c->mixed ? sizeof(float) : sizeof(double);
+    (void)NX;
+    if ((rc = c->allocn(&c->ctl, 1)) || (rc = c->allocn(&c->U, 11 * KB)) || (rc = c->allocn(&c->Ufull, 11 * c->M)) ||
+        (rc = c->allocn(&c->Fp, 8 * KB)) || (rc = c->allocn(&c->S, 12 * c->N)) ||
+        (rc = c->alloc(&c->P1, p1 * 2 * ts)) || (rc = c->alloc(&c->P2, p2 * 2 * ts)) ||
+        (rc = c->alloc(&c->R, (size_t)nfi * c->N * ts)) || (rc = c->alloc(&c->Fr, (size_t)nfo * c->N * ts)) ||
+        (rc = c->allocn(&c->X1, c->M)) || (rc = c->allocn(&c->X2, c->M)) || (rc = c->allocn(&c->Xr, c->N)) ||
+        (rc = c->allocn(&c->red, (size_t)c->nblk_red * 16)) || (rc = c->allocn(&c->dave, 8)) ||
+        (rc = c->allocn(&c->dbits, 1)))
+        return bail(rc);
+    if ((rc = reset_ctl(c, 0.0, 0.0, false))) return bail(rc);
+    *out = c;
+    return LAPS_OK;
+}
+
+void laps_destroy(laps_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->p.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+int laps_set_state(laps_ctx* c, const double* uu, const double* uu_prim) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (!uu || !uu_prim) return fail(LAPS_EINVAL, "null state pointer");
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(c->S, uu, 8 * c->N * sizeof(double), hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(c->S + 8 * c->N, uu_prim, 4 * c->N * sizeof(double), hipMemcpyHostToDevice));
+    for (int f = 0; f < 8; ++f)
+        if ((rc = fwd_fp64(c, c->S + f * c->N, c->Ufull + f * c->M))) return rc;
+    LAPS_CK(launch_gather_box(c->Ufull, c->U, 8, c->M, c->kb(), c->nxh, c->ny, c->nz, c->kept, c->stream));
+    if ((rc = reset_ctl(c, 0.0, 0.0, false))) return rc;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    c->fresh = true;
+    c->materialized = true;
+    c->stepped_since_set = false;
+    return LAPS_OK;
+}
+
+int laps_set_time(laps_ctx* c, double time, double dt_prev) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    Ctl h;
+    if ((rc = read_ctl(c, h))) return rc;
+    h.time = time;
+    h.dt_prev = dt_prev;
+    h.radius = c->pp.radius0 + c->pp.Ur * time;
+    h.tau = h.radius / c->pp.Ur;
+    h.r0r = c->pp.radius0 / h.radius;
+    h.ksq_updated = 1;  // evolve_radius calls update_ksquare (AEBmod.f90:70)
+    LAPS_CK(hipMemcpy(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice));
+    return LAPS_OK;
+}
+
+int laps_set_fixed_dt(laps_ctx* c, double dt) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    c->fixed_dt = dt > 0 ? dt : 0.0;
+    c->pp.fixed_dt = c->fixed_dt;
+    return LAPS_OK;
+}
+
+int laps_step(laps_ctx* c, int nsteps) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(LAPS_EINVAL, "nsteps < 0");
+    for (int s = 0; s < nsteps; ++s) {
+        rc = c->mixed ? one_step<float>(c) : one_step<double>(c);
+        if (rc) return rc;
+        c->stepped_since_set = true;
+    }
+    return LAPS_OK;
+}
+
+int laps_sync(laps_ctx* c) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    return LAPS_OK;
+}
+
+int laps_get_scalars(laps_ctx* c, double* time, double* dt, double* radius, int64_t* istep, int32_t* status) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    Ctl h;
+    if ((rc = read_ctl(c, h))) return rc;
+    if (time) *time = h.time;
+    if (dt) *dt = h.status ? h.dt_prev : h.dt;
+    if (radius) *radius = h.radius;
+    if (istep) *istep = h.istep;
+    if (status) *status = h.status;
+    return LAPS_OK;
+}
+
+int laps_cfl(laps_ctx* c, double* dtmin_raw, double* dt_next) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if ((rc = materialize(c))) return rc;
+    LAPS_CK(launch_ctl_begin(c->ctl, c->stream));
+    LAPS_CK(launch_cfl(c->S, c->S + 8 * c->N, c->N, c->pp, c->ctl, c->stream));
+    Ctl h;
+    if ((rc = read_ctl(c, h))) return rc;
+    double raw;
+    std::memcpy(&raw, &h.dtmin_bits, 8);
+    double dt;
+    if (c->fixed_dt > 0) {
+        dt = c->fixed_dt;
+    } else {
+        const double dtmin = raw * c->p.cfl;
+        dt = h.dt_prev;
+        if (dt < 0.98 * dtmin || dt > 1.02 * dtmin) dt = dtmin;
+    }
+    if (dtmin_raw) *dtmin_raw = raw;
+    if (dt_next) *dt_next = dt;
+    return LAPS_OK;
+}
+
+int laps_get_state(laps_ctx* c, double* uu, double* uu_prim) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if ((rc = materialize(c))) return rc;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    if (uu) LAPS_CK(hipMemcpy(uu, c->S, 8 * c->N * sizeof(double), hipMemcpyDeviceToHost));
+    if (uu_prim) LAPS_CK(hipMemcpy(uu_prim, c->S + 8 * c->N, 4 * c->N * sizeof(double), hipMemcpyDeviceToHost));
+    return LAPS_OK;
+}
+
+int laps_get_spectral(laps_ctx* c, double* uhat) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!uhat) return fail(LAPS_EINVAL, "null output");
+    if ((rc = refresh_fresh(c))) return rc;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    if (c->fresh) {
+        LAPS_CK(hipMemcpy(uhat, c->Ufull, 8 * c->M * sizeof(cpx<double>), hipMemcpyDeviceToHost));
+        return LAPS_OK;
+    }
+    const Box& b = c->kept;
+    const long long KB = c->kb();
+    std::vector<cpx<double>> box((size_t)8 * KB);
+    LAPS_CK(hipMemcpy(box.data(), c->U, box.size() * sizeof(cpx<double>), hipMemcpyDeviceToHost));
+    cpx<double>* o = (cpx<double>*)uhat;
+    std::memset(o, 0, 8 * c->M * sizeof(cpx<double>));
+    for (int f = 0; f < 8; ++f)
+        for (int cz = 0; cz < b.CZ; ++cz) {
+            const int iz = (b.CZ == c->nz || cz <= (b.CZ - 1) / 2) ? cz : c->nz - (b.CZ - cz);
+            for (int cy = 0; cy < b.CY; ++cy) {
+                const int iy = (b.CY == c->ny || cy <= (b.CY - 1) / 2) ? cy : c->ny - (b.CY - cy);
+                for (int kx = 0; kx < b.KX; ++kx)
+                    o[f * c->M + ((long long)iz * c->ny + iy) * c->nxh + kx] =
+                        box[f * KB + ((long long)cz * b.CY + cy) * b.KX + kx];
+            }
+        }
+    return LAPS_OK;
+}
+
+int laps_moments(laps_ctx* c, double ave[8], double var[8], double rho_u2[3]) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if ((rc = materialize(c))) return rc;
+    const int nb = c->nblk_red;
+    LAPS_CK(launch_moments(c->S, c->S + 8 * c->N, c->N, c->red, nb, c->stream));
+    std::vector<double> part((size_t)nb * 16);
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(part.data(), c->red, part.size() * 8, hipMemcpyDeviceToHost));
+    double s[16] = {0};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 16; ++k) s[k] += part[(size_t)b * 16 + k];
+    double a[8];
+    for (int k = 0; k < 8; ++k) {
+        a[k] = s[k] / (double)c->N;
+        const double sq = s[8 + k] / (double)c->N;
+        if (ave) ave[k] = a[k];
+        if (var) var[k] = sq - a[k] * a[k];
+    }
+    // second pass needs the means on the device
+    LAPS_CK(hipMemcpy(c->dave, a, 8 * sizeof(double), hipMemcpyHostToDevice));
+    LAPS_CK(launch_moments2(c->S, c->S + 8 * c->N, c->N, c->dave, c->red, nb, c->stream));
+    const int nb2 = nb;
+    std::vector<double> part2((size_t)nb2 * 3);
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(part2.data(), c->red, part2.size() * 8, hipMemcpyDeviceToHost));
+    double r[3] = {0, 0, 0};
+    for (int b = 0; b < nb2; ++b)
+        for (int k = 0; k < 3; ++k) r[k] += part2[(size_t)b * 3 + k];
+    if (rho_u2)
+        for (int k = 0; k < 3; ++k) rho_u2[k] = r[k] / (double)c->N;
+    return LAPS_OK;
+}
+
+int laps_max_divb(laps_ctx* c, double* out) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!out) return fail(LAPS_EINVAL, "null output");
+    if ((rc = refresh_fresh(c))) return rc;
+    const unsigned long long zero = 0;
+    LAPS_CK(hipMemcpyAsync(c->dbits, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    if (c->fresh)
+        LAPS_CK(launch_divb(c->Ufull, c->M, c->full, c->tfull.view(), c->pp, c->ctl, c->dbits, c->stream));
+    else
+        LAPS_CK(launch_divb(c->U, c->kb(), c->kept, c->tkept.view(), c->pp, c->ctl, c->dbits, c->stream));
+    unsigned long long bits = 0;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(&bits, c->dbits, 8, hipMemcpyDeviceToHost));
+    std::memcpy(out, &bits, 8);
+    return LAPS_OK;
+}
+
+int laps_fft_r2c(laps_ctx* c, int nf, const double* real_in, double* spec_out) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (nf < 0 || (nf > 0 && (!real_in || !spec_out))) return fail(LAPS_EINVAL, "bad arguments");
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    for (int f = 0; f < nf; ++f) {
+        LAPS_CK(hipMemcpy(c->Xr, real_in + (size_t)f * c->N, c->N * 8, hipMemcpyHostToDevice));
+        if ((rc = fwd_fp64(c, c->Xr, c->X1))) return rc;
+        LAPS_CK(hipStreamSynchronize(c->stream));
+        LAPS_CK(hipMemcpy(spec_out + (size_t)f * 2 * c->M, c->X1, c->M * 16, hipMemcpyDeviceToHost));
+    }
+    return LAPS_OK;
+}
+
+int laps_fft_c2r(laps_ctx* c, int nf, const double* spec_in, double* real_out) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (nf < 0 || (nf > 0 && (!spec_in || !real_out))) return fail(LAPS_EINVAL, "bad arguments");
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    for (int f = 0; f < nf; ++f) {
+        LAPS_CK(hipMemcpy(c->X2, spec_in + (size_t)f * 2 * c->M, c->M * 16, hipMemcpyHostToDevice));
+        if ((rc = inv_fp64(c, c->X2, c->full, c->Xr))) return rc;
+        LAPS_CK(hipStreamSynchronize(c->stream));
+        LAPS_CK(hipMemcpy(real_out + (size_t)f * c->N, c->Xr, c->N * 8, hipMemcpyDeviceToHost));
+    }
+    return LAPS_OK;
+}
+
+int laps_timing_enable(laps_ctx* c, int enable) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if ((rc = collect_timing(c))) return rc;
+    for (int k = 0; k < K_NCLASS; ++k) {
+        c->timing.ms[k] = 0.0;
+        c->timing.bytes[k] = 0.0;
+        c->timing.launches[k] = 0;
+    }
+    c->timing.on = enable != 0;
+    return LAPS_OK;
+}
+
+int laps_timing_read(laps_ctx* c, int index, const char** name, double* total_ms, double* total_bytes,
+                     int64_t* launches) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if ((rc = collect_timing(c))) return rc;
+    if (index < 0 || index >= K_NCLASS) {
+        if (name) *name = nullptr;
+        return LAPS_OK;
+    }
+    if (name) *name = kClassNames[index];
+    if (total_ms) *total_ms = c->timing.ms[index];
+    if (total_bytes) *total_bytes = c->timing.bytes[index];
+    if (launches) *launches = c->timing.launches[index];
+    return LAPS_OK;
+}
+
+int laps_get_stream(laps_ctx* c, void** stream) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (!stream) return fail(LAPS_EINVAL, "null output");
+    *stream = (void*)c->stream;
+    return LAPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+"""ctypes binding of laps_amd/liblaps_amd.so (the C ABI of include/laps.h).
+
+Arrays are numpy arrays in the reference's Fortran layouts: real fields
+(nx,ny,nz,nvar) float64 and spectral fields (nx/2+1,ny,nz,nvar) complex128.
+The binding fails loudly when the HIP library is missing: there is no CPU
+fallback behind it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .params import LapsParams, default_params, LAPS_FP64, LAPS_MIXED, LAPS_RUNNING, LAPS_DT_STOP  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblaps_amd.so")
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+
+# every entry point declared in include/laps.h
+EXPORTS = (
+    "laps_default_params", "laps_create", "laps_destroy", "laps_last_error", "laps_set_state",
+    "laps_set_time", "laps_set_fixed_dt", "laps_cfl", "laps_step", "laps_get_scalars", "laps_get_state",
+    "laps_get_spectral", "laps_moments", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
+    "laps_timing_enable", "laps_timing_read", "laps_get_stream",
+)
+
+
+class LapsError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LapsError(f"{LIB_PATH} is not built: run `python -m laps_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.laps_default_params.argtypes = [ctypes.POINTER(LapsParams)]
+        L.laps_create.argtypes = [ctypes.POINTER(LapsParams), ctypes.POINTER(vp)]
+        L.laps_destroy.argtypes = [vp]
+        L.laps_destroy.restype = None
+        L.laps_last_error.restype = ctypes.c_char_p
+        L.laps_set_state.argtypes = [vp, _D, _D]
+        L.laps_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.laps_set_fixed_dt.argtypes = [vp, ctypes.c_double]
+        L.laps_cfl.argtypes = [vp, _D, _D]
+        L.laps_step.argtypes = [vp, ctypes.c_int]
+        L.laps_get_scalars.argtypes = [vp, _D, _D, _D, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]
+        L.laps_get_state.argtypes = [vp, _D, _D]
+        L.laps_get_spectral.argtypes = [vp, _D]
+        L.laps_moments.argtypes = [vp, _D, _D, _D]
+        L.laps_max_divb.argtypes = [vp, _D]
+        L.laps_fft_r2c.argtypes = [vp, ctypes.c_int, _D, _D]
+        L.laps_fft_c2r.argtypes = [vp, ctypes.c_int, _D, _D]
+        L.laps_sync.argtypes = [vp]
+        L.laps_timing_enable.argtypes = [vp, ctypes.c_int]
+        L.laps_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _D, _D,
+                                       ctypes.POINTER(ctypes.c_int64)]
+        L.laps_get_stream.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().laps_last_error().decode(errors="replace")
+        raise LapsError(f"laps error {rc}: {msg}")
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_D)
+
+
+class Laps:
+    """One periodic box advanced on the GPU through the C ABI."""
+
+    def __init__(self, params):
+        self.params = params
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.nxh = self.nx // 2 + 1
+        h = ctypes.c_void_p()
+        _check(lib().laps_create(ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().laps_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _real(self, nv):
+        return np.zeros((self.nx, self.ny, self.nz, nv), dtype=np.float64, order="F")
+
+    def _spec(self, nv):
+        return np.zeros((self.nxh, self.ny, self.nz, nv), dtype=np.complex128, order="F")
+
+    def set_state(self, uu, prim):
+        uu = np.asfortranarray(uu, dtype=np.float64)
+        prim = np.asfortranarray(prim, dtype=np.float64)
+        assert uu.shape == (self.nx, self.ny, self.nz, 8) and prim.shape == (self.nx, self.ny, self.nz, 4)
+        _check(lib().laps_set_state(self._h, _fptr(uu), _fptr(prim)))
+
+    def set_time(self, t, dt_prev):
+        _check(lib().laps_set_time(self._h, float(t), float(dt_prev)))
+
+    def set_fixed_dt(self, dt):
+        _check(lib().laps_set_fixed_dt(self._h, float(dt)))
+
+    def step(self, n=1):
+        _check(lib().laps_step(self._h, int(n)))
+
+    def sync(self):
+        _check(lib().laps_sync(self._h))
+
+    def scalars(self):
+        t, dt, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ist, st = ctypes.c_int64(), ctypes.c_int32()
+        _check(lib().laps_get_scalars(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(r),
+                                      ctypes.byref(ist), ctypes.byref(st)))
+        return dict(time=t.value, dt=dt.value, radius=r.value, istep=ist.value, status=st.value)
+
+    def state(self):
+        uu, pr = self._real(8), self._real(4)
+        _check(lib().laps_get_state(self._h, _fptr(uu), _fptr(pr)))
+        return uu, pr
+
+    def spectral(self):
+        u = self._spec(8)
+        _check(lib().laps_get_spectral(self._h, _fptr(u)))
+        return u
+
+    def cfl(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().laps_cfl(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def moments(self):
+        ave, var, r = np.zeros(8), np.zeros(8), np.zeros(3)
+        _check(lib().laps_moments(self._h, _fptr(ave), _fptr(var), _fptr(r)))
+        return ave, var, r
+
+    def max_divb(self):
+        d = ctypes.c_double()
+        _check(lib().laps_max_divb(self._h, ctypes.byref(d)))
+        return d.value
+
+    def timing_enable(self, on=True):
+        _check(lib().laps_timing_enable(self._h, 1 if on else 0))
+
+    def timing(self):
+        """{kernel class: (total_ms, algorithmic_bytes, launches)} since timing_enable."""
+        out = {}
+        i = 0
+        while True:
+            name = ctypes.c_char_p()
+            ms, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+            _check(lib().laps_timing_read(self._h, i, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(by),
+                                          ctypes.byref(n)))
+            if not name.value:
+                return out
+            out[name.value.decode()] = (ms.value, by.value, n.value)
+            i += 1
+
+    def fft_r2c(self, real):
+        real = np.asfortranarray(real, dtype=np.float64)
+        nf = real.shape[3] if real.ndim == 4 else 1
+        out = self._spec(nf)
+        _check(lib().laps_fft_r2c(self._h, nf, _fptr(real), _fptr(out)))
+        return out if real.ndim == 4 else out[..., 0]
+
+    def fft_c2r(self, spec):
+        spec = np.asfortranarray(spec, dtype=np.complex128)
+        nf = spec.shape[3] if spec.ndim == 4 else 1
+        out = self._real(nf)
+        _check(lib().laps_fft_c2r(self._h, nf, _fptr(spec), _fptr(out)))
+        return out if spec.ndim == 4 else out[..., 0]

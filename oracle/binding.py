@@ -1,0 +1,200 @@
+"""ctypes binding of oracle/liblaps_oracle.so (test infrastructure only).
+
+The oracle restates the reference LAPS step in plain C (laps_oracle.c); this
+module only marshals numpy arrays in the reference's Fortran layouts:
+real (nx,ny,nz,nvar) and spectral (nx/2+1,ny,nz,nvar) complex128.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from laps_amd.params import LapsParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblaps_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp = ctypes.c_void_p
+        L.oracle_create.restype = vp
+        L.oracle_create.argtypes = [ctypes.POINTER(LapsParams)]
+        L.oracle_destroy.argtypes = [vp]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        for name in ("oracle_set_state",):
+            getattr(L, name).argtypes = [vp, _D, _D]
+        L.oracle_set_spectral.argtypes = [vp, _D]
+        L.oracle_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.oracle_set_fixed_dt.argtypes = [vp, ctypes.c_double]
+        L.oracle_step.argtypes = [vp, ctypes.c_int]
+        L.oracle_get_scalars.argtypes = [vp, _D, _D, _D, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int32)]
+        L.oracle_get_state.argtypes = [vp, _D, _D]
+        L.oracle_get_spectral.argtypes = [vp, _D]
+        L.oracle_cfl.argtypes = [vp, _D, _D]
+        L.oracle_moments.argtypes = [vp, _D, _D, _D]
+        L.oracle_max_divb.argtypes = [vp, _D]
+        L.oracle_rhs.argtypes = [vp, _D]
+        L.oracle_flux.argtypes = [vp, _D]
+        L.oracle_dealias_apply.argtypes = [vp, _D]
+        L.oracle_filters.argtypes = [vp, _D, _D, _D]
+        L.oracle_ksquare.argtypes = [vp, _D]
+        L.oracle_fft1d.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
+        L.oracle_fft_r2c.argtypes = [vp, ctypes.c_int, _D, _D]
+        L.oracle_fft_c2r.argtypes = [vp, ctypes.c_int, _D, _D]
+        L.oracle_cons_from_prim.argtypes = [ctypes.POINTER(LapsParams), ctypes.c_int64, _D, _D]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    assert a.flags["F_CONTIGUOUS"] or a.ndim == 1
+    return a.ctypes.data_as(_D)
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def fft1d(x, sign):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.empty_like(x)
+    lib().oracle_fft1d(x.size, int(sign), x.ctypes.data_as(_D), out.ctypes.data_as(_D))
+    return out
+
+
+def cons_from_prim(params, prim8):
+    """initial_calc_conserve_variable: (nx,ny,nz,8) primitive -> (uu, uu_prim)."""
+    uu = np.array(prim8, dtype=np.float64, order="F", copy=True)
+    n = uu.shape[0] * uu.shape[1] * uu.shape[2]
+    pr = np.zeros(uu.shape[:3] + (4,), dtype=np.float64, order="F")
+    lib().oracle_cons_from_prim(ctypes.byref(params), n, _ptr(uu), _ptr(pr))
+    return uu, pr
+
+
+class Oracle:
+    """One periodic box stepped by the CPU restatement."""
+
+    def __init__(self, params):
+        self.params = params
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.nxh = self.nx // 2 + 1
+        self._h = lib().oracle_create(ctypes.byref(params))
+        if not self._h:
+            raise MemoryError("oracle_create failed")
+
+    def close(self):
+        if self._h:
+            lib().oracle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _real(self, nv):
+        return np.zeros((self.nx, self.ny, self.nz, nv), dtype=np.float64, order="F")
+
+    def _spec(self, nv):
+        return np.zeros((self.nxh, self.ny, self.nz, nv), dtype=np.complex128, order="F")
+
+    def set_state(self, uu, prim):
+        uu = np.asfortranarray(uu, dtype=np.float64)
+        prim = np.asfortranarray(prim, dtype=np.float64)
+        lib().oracle_set_state(self._h, _ptr(uu), _ptr(prim))
+
+    def set_spectral(self, uhat):
+        uhat = np.asfortranarray(uhat, dtype=np.complex128)
+        lib().oracle_set_spectral(self._h, uhat.ctypes.data_as(_D))
+
+    def set_time(self, t, dt_prev):
+        lib().oracle_set_time(self._h, float(t), float(dt_prev))
+
+    def set_fixed_dt(self, dt):
+        lib().oracle_set_fixed_dt(self._h, float(dt))
+
+    def step(self, n=1):
+        lib().oracle_step(self._h, int(n))
+
+    def scalars(self):
+        t, dt, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ist, st = ctypes.c_int64(), ctypes.c_int32()
+        lib().oracle_get_scalars(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(r),
+                                 ctypes.byref(ist), ctypes.byref(st))
+        return dict(time=t.value, dt=dt.value, radius=r.value, istep=ist.value, status=st.value)
+
+    def state(self):
+        uu, pr = self._real(8), self._real(4)
+        lib().oracle_get_state(self._h, _ptr(uu), _ptr(pr))
+        return uu, pr
+
+    def spectral(self):
+        u = self._spec(8)
+        lib().oracle_get_spectral(self._h, u.ctypes.data_as(_D))
+        return u
+
+    def cfl(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        lib().oracle_cfl(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def moments(self):
+        ave, var, r = np.zeros(8), np.zeros(8), np.zeros(3)
+        lib().oracle_moments(self._h, _ptr(ave), _ptr(var), _ptr(r))
+        return ave, var, r
+
+    def max_divb(self):
+        d = ctypes.c_double()
+        lib().oracle_max_divb(self._h, ctypes.byref(d))
+        return d.value
+
+    def rhs(self):
+        f = self._spec(8)
+        lib().oracle_rhs(self._h, f.ctypes.data_as(_D))
+        return f
+
+    def flux(self):
+        f = self._real(19)
+        lib().oracle_flux(self._h, _ptr(f))
+        return f
+
+    def dealias(self, uhat):
+        u = np.array(uhat, dtype=np.complex128, order="F", copy=True)
+        lib().oracle_dealias_apply(self._h, u.ctypes.data_as(_D))
+        return u
+
+    def filters(self):
+        fx, fy, fz = np.zeros(self.nxh), np.zeros(self.ny), np.zeros(self.nz)
+        lib().oracle_filters(self._h, _ptr(fx), _ptr(fy), _ptr(fz))
+        return fx, fy, fz
+
+    def ksquare(self):
+        k = np.zeros((self.nxh, self.ny, self.nz), order="F")
+        lib().oracle_ksquare(self._h, k.ctypes.data_as(_D))
+        return k
+
+    def fft_r2c(self, real):
+        real = np.asfortranarray(real, dtype=np.float64)
+        nf = real.shape[3] if real.ndim == 4 else 1
+        out = self._spec(nf)
+        lib().oracle_fft_r2c(self._h, nf, _ptr(real), out.ctypes.data_as(_D))
+        return out if real.ndim == 4 else out[..., 0]
+
+    def fft_c2r(self, spec):
+        spec = np.asfortranarray(spec, dtype=np.complex128)
+        nf = spec.shape[3] if spec.ndim == 4 else 1
+        out = self._real(nf)
+        lib().oracle_fft_c2r(self._h, nf, spec.ctypes.data_as(_D), _ptr(out))
+        return out if spec.ndim == 4 else out[..., 0]

@@ -1,0 +1,182 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+FP64: relative L2 of the spectral state <= 1e-11 after N steps (the bound
+of the FP64 north-star target); mixed precision: <= 1e-4 against the FP64
+oracle. Sizes are those the oracle finishes in seconds.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from laps_amd.params import LAPS_DT_STOP, LAPS_MIXED
+from tests.helpers import params, rel_l2, seeded_state, TWO_PI
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11
+MIXED_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def laps_mod():
+    from laps_amd import runtime
+    runtime.lib()
+    return runtime
+
+
+def check_moments(mg, mo, rtol):
+    # variances are <f^2> - <f>^2 (mhdrms.f90:119): compare them on the scale of <f^2>
+    ag, vg, rg = mg
+    ao, vo, ro = mo
+    np.testing.assert_allclose(ag, ao, rtol=rtol, atol=1e-15)
+    scale = np.maximum(ao ** 2 + np.abs(vo), 1e-30)
+    assert np.all(np.abs(vg - vo) <= rtol * scale)
+    np.testing.assert_allclose(rg, ro, rtol=rtol * 100, atol=1e-15)
+
+
+def run_pair(laps_mod, oracle_mod, p, nsteps, seed=0, kmax=None, fixed_dt=None, **ickw):
+    uu, pr = seeded_state(p, seed=seed, kmax=kmax, **ickw)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p)
+    g.set_state(uu, pr)
+    o.set_state(uu, pr)
+    if fixed_dt:
+        g.set_fixed_dt(fixed_dt)
+        o.set_fixed_dt(fixed_dt)
+    g.step(nsteps)
+    o.step(nsteps)
+    return g, o
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 64), (64, 32, 16), (128, 16, 16), (16, 16, 256)])
+def test_fft_r2c_c2r_vs_oracle(laps_mod, oracle_mod, shape):
+    p = params(shape)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(7)
+    f = np.asfortranarray(rng.standard_normal(shape + (2,)))
+    sg, so = g.fft_r2c(f), o.fft_r2c(f)
+    assert rel_l2(sg, so) < 1e-14
+    assert rel_l2(g.fft_c2r(so), o.fft_c2r(so)) < 1e-14
+
+
+def test_fft_512_line(laps_mod, oracle_mod):
+    shape = (512, 16, 16)
+    p = params(shape)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(8)
+    f = np.asfortranarray(rng.standard_normal(shape))
+    assert rel_l2(g.fft_r2c(f), o.fft_r2c(f)) < 1e-14
+    shape = (16, 512, 16)
+    p = params(shape)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p)
+    f = np.asfortranarray(rng.standard_normal(shape))
+    assert rel_l2(g.fft_r2c(f), o.fft_r2c(f)) < 1e-14
+
+
+def test_set_state_spectrum_and_diagnostics(laps_mod, oracle_mod):
+    p = params(32, if_hall=1, ion_inertial_length=0.05)
+    uu, pr = seeded_state(p, seed=1)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p)
+    g.set_state(uu, pr)
+    o.set_state(uu, pr)
+    assert rel_l2(g.spectral(), o.spectral()) < 1e-14
+    ug, pg = g.state()
+    assert np.array_equal(ug, uu) and np.array_equal(pg, pr)
+    rg, dg = g.cfl()
+    ro, do = o.cfl()
+    assert rg == pytest.approx(ro, rel=1e-14) and dg == pytest.approx(do, rel=1e-14)
+    check_moments(g.moments(), o.moments(), 1e-12)
+    assert g.max_divb() == pytest.approx(o.max_divb(), rel=1e-6, abs=1e-15)
+
+
+@pytest.mark.parametrize("case", [
+    dict(),                                                        # config-0 physics at 32^3
+    dict(if_hall=1, ion_inertial_length=0.05),                     # Hall
+    dict(dealias_option=2),                                        # compact filter
+    dict(if_visc_exp=1, if_resis_exp=1, if_conserve_background=1),  # explicit dissipation
+    dict(if_aeb=1, radius0=1.0, Ur0=0.5, if_hall=1, ion_inertial_length=0.02),
+    dict(if_aeb=1, if_corotating=1, corotating_angle=0.3, radius0=1.0, Ur0=0.5),
+    dict(dealias_option=0, if_visc=0, if_resis=0),
+])
+def test_steps_fp64_vs_oracle(laps_mod, oracle_mod, case):
+    p = params(32, **case)
+    g, o = run_pair(laps_mod, oracle_mod, p, 5, seed=2)
+    sg, so = g.scalars(), o.scalars()
+    assert sg["istep"] == so["istep"] == 5
+    assert sg["time"] == pytest.approx(so["time"], rel=1e-13)
+    assert sg["dt"] == pytest.approx(so["dt"], rel=1e-13)
+    assert sg["radius"] == pytest.approx(so["radius"], rel=1e-14)
+    err = rel_l2(g.spectral(), o.spectral())
+    print(f"case={case} rel_l2={err:.3e}")
+    assert err < FP64_TOL
+    ug, pg = g.state()
+    uo, po = o.state()
+    assert rel_l2(ug, uo) < FP64_TOL and rel_l2(pg, po) < 1e-9
+
+
+def test_noncubic_fp64(laps_mod, oracle_mod):
+    p = params((64, 32, 16), if_hall=1, ion_inertial_length=0.05, Ly=3.0, Lz=1.5)
+    g, o = run_pair(laps_mod, oracle_mod, p, 3, seed=3, kmax=4)
+    assert rel_l2(g.spectral(), o.spectral()) < FP64_TOL
+
+
+def test_config0_64cube_100_steps(laps_mod, oracle_mod):
+    """Plain FP64 MHD, 64^3, shells 1-8 k^-5/3, rms 0.1, nu=eta=1e-3, 2/3 rule, CFL 0.4, 100 steps."""
+    p = params(64, viscosity=1e-3, resistivity=1e-3, cfl=0.4)
+    g, o = run_pair(laps_mod, oracle_mod, p, 100, seed=0, kmax=8)
+    err = rel_l2(g.spectral(), o.spectral())
+    sg, so = g.scalars(), o.scalars()
+    print(f"64^3 100 steps: rel_l2={err:.3e} t={sg['time']:.6f} dt={sg['dt']:.6e}")
+    assert sg["istep"] == 100
+    assert sg["time"] == pytest.approx(so["time"], rel=1e-12)
+    assert err < FP64_TOL
+    assert g.max_divb() < 1e-12
+    check_moments(g.moments(), o.moments(), 1e-10)
+
+
+@pytest.mark.parametrize("case", [dict(), dict(if_hall=1, ion_inertial_length=0.05), dict(dealias_option=2)])
+def test_mixed_precision_vs_fp64_oracle(laps_mod, oracle_mod, case):
+    p = params(32, precision=LAPS_MIXED, **case)
+    g = laps_mod.Laps(p)
+    p64 = params(32, **case)
+    o = oracle_mod.Oracle(p64)
+    uu, pr = seeded_state(p64, seed=4)
+    g.set_state(uu, pr)
+    o.set_state(uu, pr)
+    g.step(20)
+    o.step(20)
+    err = rel_l2(g.spectral(), o.spectral())
+    print(f"mixed case={case} rel_l2={err:.3e}")
+    assert err < MIXED_TOL
+    assert g.scalars()["dt"] == pytest.approx(o.scalars()["dt"], rel=1e-4)
+
+
+def test_dt_stop_leaves_state(laps_mod, oracle_mod):
+    # a CFL factor of 1e-12 forces dt < 1e-8: the step is refused (mhd.f90:216-242)
+    p = params(16, cfl=1e-12)
+    g, o = run_pair(laps_mod, oracle_mod, p, 1, seed=5)
+    sg, so = g.scalars(), o.scalars()
+    assert sg["status"] == so["status"] == LAPS_DT_STOP
+    assert sg["istep"] == 0 and sg["time"] == 0.0
+    uu, pr = seeded_state(p, seed=5)
+    assert np.array_equal(g.state()[0], uu)
+
+
+def test_fixed_dt_and_restart_time(laps_mod, oracle_mod):
+    p = params(32, if_aeb=1, radius0=1.0, Ur0=0.5)
+    uu, pr = seeded_state(p, seed=6)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p)
+    for x in (g, o):
+        x.set_state(uu, pr)
+        x.set_time(0.25, 0.0)
+        x.set_fixed_dt(0.01)
+        x.step(4)
+    assert g.scalars()["time"] == pytest.approx(0.29, rel=1e-14)
+    assert g.scalars()["radius"] == pytest.approx(1.0 + 0.5 * 0.29, rel=1e-14)
+    assert rel_l2(g.spectral(), o.spectral()) < FP64_TOL

@@ -1,0 +1,275 @@
+"""Pins the CPU oracle (oracle/laps_oracle.c) before it is trusted.
+
+Each check is chosen so that a plausible restatement mistake (a dropped
+term, a wrong sign or index, a transposed operand) fails it: library
+routines (numpy.fft), brute-force sums, closed forms of the equations, and
+invariants. The reference ships no tests or golden vectors for this path.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.helpers import params, rel_l2, seeded_state, TWO_PI
+
+
+def brute_dft(x, sign):
+    n = x.size
+    k = np.arange(n)
+    return np.exp(sign * 2j * np.pi * np.outer(k, k) / n) @ x
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 12, 16, 24, 48])
+def test_fft1d_brute_force(oracle_mod, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for sign in (-1, 1):
+        got = oracle_mod.fft1d(x, sign)
+        assert np.abs(got - brute_dft(x, sign)).max() < 1e-13 * max(1, n)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 96, 128, 256, 512])
+def test_fft1d_numpy(oracle_mod, n):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel_l2(oracle_mod.fft1d(x, -1), np.fft.fft(x)) < 1e-15
+    assert rel_l2(oracle_mod.fft1d(x, +1), np.fft.ifft(x) * n) < 1e-15
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 8), (8, 32, 16), (96, 16, 32)])
+def test_fft3d_r2c_c2r_numpy(oracle_mod, shape):
+    p = params(shape)
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(2)
+    f = np.asfortranarray(rng.standard_normal(shape + (2,)))
+    s = o.fft_r2c(f)
+    n = shape[0] * shape[1] * shape[2]
+    for c in range(2):
+        ref = np.fft.rfftn(f[..., c].T).T / n      # fftw.f90: each axis divided by its length
+        assert rel_l2(s[..., c], ref) < 1e-15
+    back = o.fft_c2r(s)                            # unnormalised inverse
+    assert rel_l2(back, f) < 1e-14
+    # c2r reads only Re of DC / Nyquist coefficients (FFTW c2r convention)
+    s2 = s.copy()
+    s2[0, 0, 0, :] += 1j * 5.0
+    s2[shape[0] // 2, 0, 0, :] += 1j * 7.0
+    assert rel_l2(o.fft_c2r(s2), back) < 1e-14
+
+
+def test_single_mode_transform(oracle_mod):
+    # cos(3x + 2y - z) has coefficient 1/2 at (3,2,-1) only (normalised forward)
+    p = params(16)
+    o = oracle_mod.Oracle(p)
+    x = np.arange(16) * TWO_PI / 16
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    s = o.fft_r2c(np.cos(3 * X + 2 * Y - Z))
+    expect = np.zeros_like(s)
+    expect[3, 2, 16 - 1] = 0.5
+    assert np.abs(s - expect).max() < 1e-15
+
+
+def test_dealias_sphere_count(oracle_mod):
+    # option 1 keeps sqrt((mx/nx)^2+(my/ny)^2+(mz/nz)^2) <= 1/3 (dealiasing.f90:152-164)
+    p = params(64)
+    o = oracle_mod.Oracle(p)
+    u = np.ones((33, 64, 64, 8), dtype=complex, order="F")
+    d = o.dealias(u)
+    kept = np.count_nonzero(d[..., 0])
+    m = np.fft.fftfreq(64, 1 / 64)
+    mx = np.arange(33)[:, None, None]
+    rr = np.sqrt((mx / 64.0) ** 2 + (m[None, :, None] / 64.0) ** 2 + (m[None, None, :] / 64.0) ** 2)
+    assert kept == np.count_nonzero(rr <= 1 / 3) == 21042
+    assert np.all(d[..., 0][d[..., 0] != 0] == 1)
+
+
+def test_compact_filter_closed_form(oracle_mod):
+    p = params(32, dealias_option=2)
+    o = oracle_mod.Oracle(p)
+    fx, fy, fz = o.filters()
+    assert fx[0] == 1.0 and fy[0] == 1.0
+    # theta = pi: numerator a - b + c = 0 exactly, round-off / (1 - 2a) = 0.01
+    assert abs(fx[16]) < 1e-13 and abs(fy[16]) < 1e-13
+    a = 0.495
+    th = np.arange(17) * TWO_PI / 32
+    ref = ((5 + 6 * a) / 8 + (1 + 2 * a) / 2 * np.cos(th) - (1 - 2 * a) / 8 * np.cos(2 * th)) / (1 + 2 * a * np.cos(th))
+    assert np.abs(fx - ref).max() < 1e-13
+    assert np.all(np.diff(fx) <= 0)
+
+
+def uniform_prim(n, rho=1.0, u=(0, 0, 0), b=(1, 0, 0), pr=1.0):
+    prim = np.zeros((n, n, n, 8), order="F")
+    prim[..., 0] = rho
+    prim[..., 1:4] = u
+    prim[..., 4:7] = b
+    prim[..., 7] = pr
+    return prim
+
+
+def test_cfl_closed_form(oracle_mod):
+    # rho=1, p=1, B=(1,0,0), u=0: c_fast,x = sqrt(gamma), c_fast,y = c_fast,z = sqrt(gamma+1)
+    p = params(16, cfl=0.4)
+    o = oracle_mod.Oracle(p)
+    uu, pr = oracle_mod.cons_from_prim(p, uniform_prim(16))
+    o.set_state(uu, pr)
+    raw, dt = o.cfl()
+    g = p.adiabatic_index
+    dx = TWO_PI / 16
+    assert raw == pytest.approx(min(dx / math.sqrt(g), dx / math.sqrt(g + 1)), rel=1e-14)
+    assert dt == pytest.approx(0.4 * raw, rel=1e-14)
+    # B = 0: c_fast = c_s, c_slow = 0, with |u| added
+    uu, pr = oracle_mod.cons_from_prim(p, uniform_prim(16, u=(0.5, 0, 0), b=(0, 0, 0)))
+    o.set_state(uu, pr)
+    raw, _ = o.cfl()
+    assert raw == pytest.approx(dx / (0.5 + math.sqrt(g)), rel=1e-14)
+
+
+def test_cfl_hall_signed_max(oracle_mod):
+    # Hall speed uses the signed max of (Bx,By,Bz) (mhd.f90:396-398): all-negative B gives no Hall limit
+    p = params(16, if_hall=1, ion_inertial_length=5.0)
+    o = oracle_mod.Oracle(p)
+    dx = TWO_PI / 16
+    g = p.adiabatic_index
+    uu, pr = oracle_mod.cons_from_prim(p, uniform_prim(16, b=(1.0, 0, 0)))
+    o.set_state(uu, pr)
+    assert o.cfl()[0] == pytest.approx(dx / (5.0 * 1.0 / dx), rel=1e-14)
+    uu, pr = oracle_mod.cons_from_prim(p, uniform_prim(16, b=(-1.0, -1e-3, -1e-3)))
+    o.set_state(uu, pr)
+    assert o.cfl()[0] < dx / math.sqrt(g) * 1.0000001
+
+
+def rk3_poly(z):
+    return 1 + z + z * z / 2 + z ** 3 / 6
+
+
+@pytest.mark.parametrize("implicit", [False, True])
+def test_rk3_viscous_shear_mode(oracle_mod, implicit):
+    # u_y = eps sin(x) with uniform rho, p, B = 0 has a linear RHS up to O(eps^2)
+    # (viscous heating feeds p, hence u_x, at second order)
+    # d(rho u_y)/dt = -nu k^2 rho u_y; explicit: amplification 1+z+z^2/2+z^3/6,
+    # implicit: prod 1/(1 + tau_k nu k^2), tau = (8/15, 2/15, 1/3) dt (rktmod.f90:17-61)
+    n, nu, dt, eps = 16, 0.5, 0.2, 1e-7
+    p = params(n, if_visc=1, viscosity=nu, if_visc_exp=0 if implicit else 1, if_resis=0, dealias_option=1)
+    o = oracle_mod.Oracle(p)
+    x = np.arange(n) * TWO_PI / n
+    prim = uniform_prim(n, b=(0, 0, 0))
+    prim[..., 2] = eps * np.sin(x)[:, None, None]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    o.set_fixed_dt(dt)
+    u0 = o.spectral()[1, 0, 0, 2]
+    o.step(1)
+    u1 = o.spectral()[1, 0, 0, 2]
+    a = nu * 1.0 * dt
+    amp = 1 / ((1 + 8 / 15 * a) * (1 + 2 / 15 * a) * (1 + a / 3)) if implicit else rk3_poly(-a)
+    assert abs(u1 / u0 - amp) < 1e-13
+    assert abs(rk3_poly(-a) - math.exp(-a)) > 1e-12  # the check distinguishes the scheme
+
+
+def test_rhs_advection_single_mode(oracle_mod):
+    # rho = 1 + eps cos(x), u = (U,0,0): d rho/dt = -U d rho/dx -> fnl1(k=1) = -i U rho_hat
+    n, U, eps = 16, 0.3, 1e-2
+    p = params(n, if_visc=0, if_resis=0)
+    o = oracle_mod.Oracle(p)
+    x = np.arange(n) * TWO_PI / n
+    prim = uniform_prim(n, u=(U, 0, 0))
+    prim[..., 0] = 1 + eps * np.cos(x)[:, None, None]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    f = o.rhs()
+    rho_hat = o.spectral()[1, 0, 0, 0]
+    assert abs(rho_hat - eps / 2) < 1e-16
+    assert abs(f[1, 0, 0, 0] - (-1j * U * rho_hat)) < 1e-15
+    mask = np.ones(f.shape[:3], bool)
+    mask[1, 0, 0] = False
+    assert np.abs(f[..., 0][mask]).max() < 1e-15
+
+
+def test_rhs_hall_whistler_sign(oracle_mod):
+    # B = (b sin z, 0, B0), u = 0, rho = 1: E_Hall = d_i J x B and dB/dt = -curl E
+    # gives dBy/dt = d_i B0 b k^2 sin(kz) -> fnl(By) at mz=+1 equals -i d_i B0 b / 2
+    n, b, B0, di = 16, 1e-3, 1.0, 0.2
+    p = params(n, if_hall=1, ion_inertial_length=di, if_visc=0, if_resis=0)
+    o = oracle_mod.Oracle(p)
+    z = np.arange(n) * TWO_PI / n
+    prim = uniform_prim(n, b=(0, 0, B0))
+    prim[..., 4] = b * np.sin(z)[None, None, :]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    f = o.rhs()
+    assert abs(f[0, 0, 1, 5] - (-1j * di * B0 * b / 2)) < 1e-16
+    assert abs(f[0, 0, n - 1, 5] - (1j * di * B0 * b / 2)) < 1e-16
+    assert np.abs(f[..., 4]).max() < 1e-17 and np.abs(f[..., 6]).max() < 1e-17
+    p0 = params(n, if_hall=0, if_visc=0, if_resis=0)
+    o0 = oracle_mod.Oracle(p0)
+    o0.set_state(uu, pr)
+    assert np.abs(o0.rhs()[..., 5]).max() < 1e-17    # no Hall: By stays
+
+
+def test_aeb_uniform_state_decay(oracle_mod):
+    # uniform state with expansion: rho ~ R^-2, rho u_x ~ R^-2, rho u_y,z ~ R^-3,
+    # Bx ~ R^-2, By,z ~ R^-1 (notes eq. 6-10); tau frozen per step (mhd.f90:248)
+    n, R0, Ur, dt = 16, 1.0, 0.5, 0.01
+    p = params(n, if_aeb=1, radius0=R0, Ur0=Ur, if_visc=0, if_resis=0)
+    o = oracle_mod.Oracle(p)
+    prim = uniform_prim(n, u=(0.1, 0.2, 0.3), b=(1.0, 0.5, 0.25))
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    o.set_fixed_dt(dt)
+    u0 = o.spectral()[0, 0, 0, :]
+    tau = R0 / Ur
+    o.step(1)
+    u1 = o.spectral()[0, 0, 0, :]
+    rates = np.array([2, 2, 3, 3, 2, 1, 1])
+    for v in range(7):
+        assert abs(u1[v] / u0[v] - rk3_poly(-rates[v] * dt / tau)) < 1e-13
+    # energy: its source term is the adiabatic/kinetic/magnetic decay of each part
+    _, prm = o.state()
+    g = p.adiabatic_index
+    assert prm[0, 0, 0, 3] == pytest.approx(1.0 * math.exp(-2 * g * dt / tau), rel=1e-6)
+    assert o.scalars()["radius"] == pytest.approx(R0 + Ur * dt, rel=1e-15)
+
+
+def test_invariants_seeded(oracle_mod):
+    p = params(16, if_hall=1, ion_inertial_length=0.05)
+    o = oracle_mod.Oracle(p)
+    uu, pr = seeded_state(p, seed=3, kmax=3)
+    o.set_state(uu, pr)
+    s0 = o.spectral()
+    o.step(3)
+    s1 = o.spectral()
+    assert np.array_equal(s1[0, 0, 0, :], s0[0, 0, 0, :])     # k = 0 modes unchanged without AEB
+    assert o.max_divb() < 1e-14                                # solenoidal IC stays solenoidal
+    # total energy has no dissipative flux: <e> conserved to round-off
+    assert abs(s1[0, 0, 0, 7].real - s0[0, 0, 0, 7].real) < 1e-14
+    # kx = 0 plane Hermitian to round-off
+    pl = s1[0, :, :, :]
+    herm = np.conj(pl[(-np.arange(16)) % 16][:, (-np.arange(16)) % 16])
+    assert np.abs(pl - herm).max() < 1e-15
+
+
+def test_moments_bruteforce(oracle_mod):
+    p = params(16)
+    o = oracle_mod.Oracle(p)
+    uu, pr = seeded_state(p, seed=4, kmax=3)
+    o.set_state(uu, pr)
+    ave, var, ru2 = o.moments()
+    v = np.stack([uu[..., 0], pr[..., 0], pr[..., 1], pr[..., 2], uu[..., 4], uu[..., 5], uu[..., 6], pr[..., 3]], -1)
+    assert np.allclose(ave, v.reshape(-1, 8).mean(0), rtol=1e-13, atol=1e-15)
+    assert np.allclose(var, (v.reshape(-1, 8) ** 2).mean(0) - v.reshape(-1, 8).mean(0) ** 2, rtol=1e-10, atol=1e-15)
+    for i in range(3):
+        assert ru2[i] == pytest.approx((uu[..., 0] * (pr[..., i] - ave[1 + i]) ** 2).mean(), rel=1e-12)
+
+
+def test_divb_bruteforce(oracle_mod):
+    p = params(16)
+    o = oracle_mod.Oracle(p)
+    rng = np.random.default_rng(5)
+    prim = uniform_prim(16)
+    prim[..., 4:7] += 1e-2 * rng.standard_normal((16, 16, 16, 3))
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    s = o.spectral()
+    m = np.fft.fftfreq(16, 1 / 16)
+    kx = np.arange(9)[:, None, None]
+    d = 1j * (kx * s[..., 4] + m[None, :, None] * s[..., 5] + m[None, None, :] * s[..., 6])
+    assert o.max_divb() == pytest.approx(np.abs(d).max(), rel=1e-13)
