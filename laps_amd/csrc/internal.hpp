@@ -81,12 +81,28 @@ hipError_t launch_x_c2r(int n, const cpx<T>* in, T* out, const XArgs& a, const c
 template <typename T>
 hipError_t launch_x_r2c(int n, const T* in, cpx<T>* out, const XArgs& a, const cpx<T>* tw, hipStream_t s);
 
+// --- fused x pass (xfused.hip) ---
+struct XFArgs {
+    long long nrows;          // nz*ny rows per field
+    long long npts;           // nx*ny*nz
+    const void* st;           // state spectral rows (T complex), kx_s per row
+    int kx_s;
+    long long fs_s;           // field stride of st (elements)
+    const void* jb;           // Hall current spectral rows, kx_j per row
+    int kx_j;
+    long long fs_j;
+    const double* S;          // first substage: uploaded uu(8) + uu_prim(4), FP64 real
+    void* out;                // flux spectral rows (T complex), kx_o per row
+    int kx_o;
+    long long fs_o;
+};
+template <typename T>
+hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp,
+                         Ctl* ctl, hipStream_t s);
+
 // --- physics launchers (physics_kernels.hip) ---
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fstride, Box b, BoxTables t,
                        const PhysParams& pp, const Ctl* ctl, hipStream_t s);
-template <typename T>
-hipError_t launch_flux(const T* R, const double* IC, T* F, long long npts, int nflux, bool first, bool cfl,
-                       const PhysParams& pp, Ctl* ctl, hipStream_t s);
 template <typename T>
 hipError_t launch_rhs_rk(const cpx<T>* E, cpx<double>* U, cpx<double>* Fp, Box b, BoxTables t, int irk,
                          const PhysParams& pp, const Ctl* ctl, hipStream_t s);

@@ -9,7 +9,7 @@
 //   Fp     [8][CZ][CY][KX] FP64 complex  : previous RHS fnl_rk
 //   S      [12][nz][ny][nx] FP64 real    : real-space state uu(8)+uu_prim(4)
 //                                         (the uploaded one, or materialised)
-//   P1,P2  T complex, R,Fr T real        : transform intermediates, T = FP64
+//   P1,P2  T complex                     : transform intermediates, T = FP64
 //                                         or FP32 (mixed precision)
 // The "kept box" holds kx < KX and the stored ky/kz rows of the modes that
 // survive dealiasing: all modes for the compact filter (option 2), the
@@ -99,10 +99,10 @@ struct DevTables {
 
 // Kernel classes of the step pipeline, timed with HIP events on the
 // context's stream when laps_timing_enable() is on.
-enum KClass { K_ZINV, K_YINV, K_XC2R, K_FLUX, K_XR2C, K_YFWD, K_ZFWD, K_RHSRK, K_CURL, K_CTL, K_NCLASS };
+enum KClass { K_ZINV, K_YINV, K_XFUSED, K_YFWD, K_ZFWD, K_RHSRK, K_CURL, K_CTL, K_XC2R, K_XR2C, K_NCLASS };
 static const char* const kClassNames[K_NCLASS] = {
-    "fft_z_inverse", "fft_y_inverse", "fft_x_c2r", "flux_cfl", "fft_x_r2c",
-    "fft_y_forward", "fft_z_forward", "rhs_rk3_dealias", "hall_curl", "control"};
+    "fft_z_inverse", "fft_y_inverse", "x_c2r_flux_r2c", "fft_y_forward", "fft_z_forward",
+    "rhs_rk3_dealias", "hall_curl", "control", "fft_x_c2r", "fft_x_r2c"};
 
 struct Timing {
     bool on = false;
@@ -132,8 +132,6 @@ struct laps_ctx {
     double* S = nullptr;
     void* P1 = nullptr;
     void* P2 = nullptr;
-    void* R = nullptr;
-    void* Fr = nullptr;
     cpx<double>* X1 = nullptr;
     cpx<double>* X2 = nullptr;
     double* Xr = nullptr;
@@ -307,6 +305,7 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 
 // spectral box (fields with stride fs) -> real fields, unnormalised
 // (transform_uu_fourier_to_real, fftw.f90:73-103)
+// With dst == nullptr the x leg is left out and B holds the y-inverted rows.
 template <typename T>
 int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
                  cpx<T>* B) {
@@ -327,6 +326,7 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
     const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
     LAPS_CK(timed(c, K_YINV, ey * (b.CY + NY),
                   [&] { return launch_strided<T, T>(c->ny, +1, A, B, y, tw_of<T>(c, 1), T(1), c->stream); }));
+    if (!dst) return LAPS_OK;
     XArgs x{};
     x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = b.KX; x.rrow = NX;
     LAPS_CK(timed(c, K_XC2R, (double)x.nrows * sizeof(T) * (2.0 * b.KX + NX),
@@ -335,14 +335,17 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
 }
 
 // real fields -> spectral box in A, normalised by 1/(nx ny nz)
-// (transform_flux_real_to_fourier, mhdrhs.f90:128-172; fftw.f90:42-71)
+// (transform_flux_real_to_fourier, mhdrhs.f90:128-172; fftw.f90:42-71).
+// With src == nullptr the x leg is left out: A already holds x-transformed rows.
 template <typename T>
 int fwd_pipeline(laps_ctx* c, const T* src, int nf, const Box& b, cpx<T>* A, cpx<T>* B) {
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
-    XArgs x{};
-    x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = b.KX; x.rrow = NX;
-    LAPS_CK(timed(c, K_XR2C, (double)x.nrows * sizeof(T) * (2.0 * b.KX + NX),
-                  [&] { return launch_x_r2c<T>(c->nx, src, A, x, tw_of<T>(c, 0), c->stream); }));
+    if (src) {
+        XArgs x{};
+        x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = b.KX; x.rrow = NX;
+        LAPS_CK(timed(c, K_XR2C, (double)x.nrows * sizeof(T) * (2.0 * b.KX + NX),
+                      [&] { return launch_x_r2c<T>(c->nx, src, A, x, tw_of<T>(c, 0), c->stream); }));
+    }
     StridedArgs y{};
     y.ncol = b.KX; y.in_cnt = (int)NY; y.out_cnt = b.CY;
     y.in_ls = b.KX; y.out_ls = b.KX;
@@ -363,17 +366,26 @@ int fwd_pipeline(laps_ctx* c, const T* src, int nf, const Box& b, cpx<T>* A, cpx
     return LAPS_OK;
 }
 
-// one substage of evolve (mhd.f90:303-325)
+// one substage of evolve (mhd.f90:303-325): [curl J] -> z, y inverse ->
+// fused x pass (C2R, fluxes, CFL, R2C) -> y, z forward -> RHS + RK3 + dealias
 template <typename T>
 int substage(laps_ctx* c, int irk) {
     const bool first = c->fresh && irk == 0;
     const bool hall = c->p.if_hall != 0;
     const long long KB = c->kb();
-    T* R = (T*)c->R;
-    T* Fr = (T*)c->Fr;
+    const long long NZ = c->nz, NY = c->ny;
     cpx<T>* P1 = (cpx<T>*)c->P1;
     cpx<T>* P2 = (cpx<T>*)c->P2;
+    const Box& k = c->kept;
     int rc;
+    XFArgs xa{};
+    xa.nrows = NZ * NY;
+    xa.npts = c->N;
+    xa.S = c->S;
+    xa.out = P1;
+    xa.kx_o = k.KX;
+    xa.fs_o = NZ * NY * k.KX;
+    double xin;
     if (first) {
         // first substage: the uploaded real fields are used as they are; J
         // comes from the un-dealiased spectrum (mhd.f90:121-122, :305)
@@ -382,26 +394,37 @@ int substage(laps_ctx* c, int irk) {
                 return launch_curl(c->Ufull, c->Ufull + 8 * c->M, c->M, c->full, c->tfull.view(), c->pp, c->ctl,
                                    c->stream);
             }));
-            if ((rc = inv_pipeline<T>(c, c->Ufull + 8 * c->M, c->M, c->full, 3, R + 8 * c->N, P1, P2))) return rc;
+            if ((rc = inv_pipeline<T>(c, c->Ufull + 8 * c->M, c->M, c->full, 3, (T*)nullptr, P1, P2))) return rc;
+            xa.jb = P2;
+            xa.kx_j = c->nxh;
+            xa.fs_j = NZ * NY * c->nxh;
         }
+        xin = 96.0 * c->N + (hall ? 3.0 * xa.nrows * c->nxh * 2 * sizeof(T) : 0.0);
     } else {
         if (hall)
             LAPS_CK(timed(c, K_CURL, 96.0 * KB, [&] {
                 return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
             }));
-        if ((rc = inv_pipeline<T>(c, c->U, KB, c->kept, c->nf_in(), R, P1, P2))) return rc;
+        if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P2))) return rc;
+        xa.st = P2;
+        xa.kx_s = k.KX;
+        xa.fs_s = NZ * NY * k.KX;
+        xa.jb = P2 + 8 * xa.fs_s;
+        xa.kx_j = k.KX;
+        xa.fs_j = xa.fs_s;
+        xin = (double)c->nf_in() * xa.nrows * k.KX * 2 * sizeof(T);
     }
     const bool cfl = irk == 0 && !(c->fixed_dt > 0);
     if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_begin(c->ctl, c->stream); }));
-    const double fin = first ? (12.0 * 8 + (hall ? 3.0 * sizeof(T) : 0.0)) : c->nf_in() * (double)sizeof(T);
-    LAPS_CK(timed(c, K_FLUX, (double)c->N * (fin + c->nf_out() * sizeof(T)), [&] {
-        return launch_flux<T>(R, c->S, Fr, c->N, c->nf_out(), first, cfl, c->pp, c->ctl, c->stream);
+    const double xout = (double)c->nf_out() * xa.nrows * k.KX * 2 * sizeof(T);
+    LAPS_CK(timed(c, K_XFUSED, xin + xout, [&] {
+        return launch_xfused<T>(c->nx, first, cfl, xa, tw_of<T>(c, 0), c->pp, c->ctl, c->stream);
     }));
     if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_finalize(c->ctl, c->pp, c->stream); }));
-    if ((rc = fwd_pipeline<T>(c, Fr, c->nf_out(), c->kept, P1, P2))) return rc;
+    if ((rc = fwd_pipeline<T>(c, (const T*)nullptr, c->nf_out(), k, P1, P2))) return rc;
     const double rb = (double)KB * (c->nf_out() * 2.0 * sizeof(T) + 16.0 * (8 + 8 + 8 + (irk ? 8 : 0)));
     LAPS_CK(timed(c, K_RHSRK, rb, [&] {
-        return launch_rhs_rk<T>(P1, c->U, c->Fp, c->kept, c->tkept.view(), irk, c->pp, c->ctl, c->stream);
+        return launch_rhs_rk<T>(P1, c->U, c->Fp, k, c->tkept.view(), irk, c->pp, c->ctl, c->stream);
     }));
     return LAPS_OK;
 }
@@ -620,7 +643,6 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     if ((rc = c->allocn(&c->ctl, 1)) || (rc = c->allocn(&c->U, 11 * KB)) || (rc = c->allocn(&c->Ufull, 11 * c->M)) ||
         (rc = c->allocn(&c->Fp, 8 * KB)) || (rc = c->allocn(&c->S, 12 * c->N)) ||
         (rc = c->alloc(&c->P1, p1 * 2 * ts)) || (rc = c->alloc(&c->P2, p2 * 2 * ts)) ||
-        (rc = c->alloc(&c->R, (size_t)nfi * c->N * ts)) || (rc = c->alloc(&c->Fr, (size_t)nfo * c->N * ts)) ||
         (rc = c->allocn(&c->X1, c->M)) || (rc = c->allocn(&c->X2, c->M)) || (rc = c->allocn(&c->Xr, c->N)) ||
         (rc = c->allocn(&c->red, (size_t)c->nblk_red * 16)) || (rc = c->allocn(&c->dave, 8)) ||
         (rc = c->allocn(&c->dbits, 1)))

@@ -1,9 +1,6 @@
 // physics_kernels.hip -- pointwise kernels of the LAPS step.
 //
 //  k_curl     : Hall current J_hat = ik x B_hat      (mhdrhs.f90:313-339)
-//  k_flux     : primitives, 18(+1) fluxes, Hall EMF, AEB energy source and,
-//               in the first substage, the CFL minimum (mhdrhs.f90:21-124,
-//               :282-294; mhd.f90:350-416)
 //  k_rhs_rk   : spectral divergence/curl, AEB damping, explicit nu/eta, the
 //               low-storage RK3 update with implicit diffusion, dealiasing
 //               (mhdrhs.f90:174-279; rktmod.f90:34-62; dealiasing.f90:70-113)
@@ -13,6 +10,7 @@
 //  reductions : CFL over a real state, moments (mhdrms.f90:53-126) and
 //               max |k.B| (mhd.f90:522-570), all in FP64.
 #include "internal.hpp"
+#include "physics_dev.hpp"
 
 namespace laps {
 
@@ -88,44 +86,6 @@ hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fs, Box b
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ CFL point
-// dt candidate of one point (mhd.f90:355-409), FP64
-__device__ __forceinline__ double cfl_point(double rho, double mx, double my, double mz, double Bx, double By,
-                                            double Bz, double p, double ux, double uy, double uz,
-                                            const PhysParams& pp, double rr, double dmin) {
-    (void)mx; (void)my; (void)mz;
-    const double cs2 = pp.gamma * p / rho;
-    const double sr = sqrt(rho);
-    const double cax = Bx / sr, cay = By / sr, caz = Bz / sr;
-    const double ca2 = cax * cax + cay * cay + caz * caz;
-    const double cms2 = cs2 + ca2;
-    const double cnx = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cax * cax, 0.0));
-    const double cny = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cay * cay, 0.0));
-    const double cnz = sqrt(fmax(cms2 * cms2 - 4 * cs2 * caz * caz, 0.0));
-    const double s2 = sqrt(2.0);
-    const double cfx = sqrt(cms2 + cnx) / s2, cfy = sqrt(cms2 + cny) / s2, cfz = sqrt(cms2 + cnz) / s2;
-    const double csx = sqrt(fmax(cms2 - cnx, 0.0)) / s2;
-    const double csy = sqrt(fmax(cms2 - cny, 0.0)) / s2;
-    const double csz = sqrt(fmax(cms2 - cnz, 0.0)) / s2;
-    auto cmax = [](double u, double cf, double cs, double ca) {
-        return fmax(fmax(fmax(fabs(u + cf), fabs(u + cs)), fmax(fabs(u + ca), fabs(u - cf))),
-                    fmax(fmax(fabs(u - cs), fabs(u - ca)), fabs(u)));
-    };
-    double cmx = cmax(ux, cfx, csx, cax), cmy = cmax(uy, cfy, csy, cay), cmz = cmax(uz, cfz, csz, caz);
-    if (pp.hall) {
-        double bmax = Bx;  // signed max of the components (mhd.f90:396-398)
-        if (By > bmax) bmax = By;
-        if (Bz > bmax) bmax = Bz;
-        const double ch = pp.di / rho * bmax / dmin;
-        cmx = fmax(cmx, ch);
-        cmy = fmax(cmy, ch);
-        cmz = fmax(cmz, ch);
-    }
-    const double dtx = pp.dx / cmx, dty = pp.dy / cmy * rr, dtz = pp.dz / cmz * rr;
-    double d = dtx < dty ? dtx : dty;
-    return dtz < d ? dtz : d;
-}
-
 // block-wide minimum of non-negative doubles, folded into *bits with an
 // unsigned 64-bit atomic (the IEEE order of non-negative doubles is the
 // integer order of their bit patterns; NaN sorts above +Inf and is skipped)
@@ -140,106 +100,6 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* b
         for (int i = 1; i < (int)(blockDim.x / 64); ++i) m = fmin(m, red[i]);
         atomicMin(bits, (unsigned long long)__double_as_longlong(m));
     }
-}
-
-// ------------------------------------------------------------------ flux
-// R: real fields rho, rho*u(3), B(3), e [, J(3)] ; IC (first substage): the
-// uploaded FP64 uu(8) and uu_prim(4) used as they are (mhd.f90:121, :303-305)
-template <typename T, bool FIRST, bool CFL, bool HALL, bool AEB>
-__global__ __launch_bounds__(kThreads) void k_flux(const T* __restrict__ R, const double* __restrict__ IC,
-                                                   T* __restrict__ F, long long N, PhysParams pp,
-                                                   Ctl* __restrict__ ctl) {
-    const T g = T(pp.gamma), di = T(pp.di);
-    const T tau = T(ctl->tau);
-    const double rr = ctl->radius / pp.radius0;
-    double dmin = pp.dx < pp.dy ? pp.dx : pp.dy;
-    if (pp.dz < dmin) dmin = pp.dz;
-    double dtloc = __longlong_as_double(0x7FF0000000000000LL);
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
-        T rho, mx, my, mz, Bx, By, Bz, e, ux, uy, uz, p;
-        if constexpr (FIRST) {
-            rho = T(IC[i]); mx = T(IC[N + i]); my = T(IC[2 * N + i]); mz = T(IC[3 * N + i]);
-            Bx = T(IC[4 * N + i]); By = T(IC[5 * N + i]); Bz = T(IC[6 * N + i]); e = T(IC[7 * N + i]);
-            ux = T(IC[8 * N + i]); uy = T(IC[9 * N + i]); uz = T(IC[10 * N + i]); p = T(IC[11 * N + i]);
-        } else {
-            rho = R[i]; mx = R[N + i]; my = R[2 * N + i]; mz = R[3 * N + i];
-            Bx = R[4 * N + i]; By = R[5 * N + i]; Bz = R[6 * N + i]; e = R[7 * N + i];
-            // update_uu_prim_from_uu (mhdrhs.f90:285-293)
-            ux = mx / rho; uy = my / rho; uz = mz / rho;
-            p = (e - T(0.5) * (mx * ux + my * uy + mz * uz + Bx * Bx + By * By + Bz * Bz)) * (g - T(1));
-        }
-        const T ptot = p + T(0.5) * (Bx * Bx + By * By + Bz * Bz);
-        const T udotb = ux * Bx + uy * By + uz * Bz;
-        F[i] = mx;
-        F[N + i] = my;
-        F[2 * N + i] = mz;
-        F[3 * N + i] = mx * ux - Bx * Bx + ptot;
-        F[4 * N + i] = my * ux - By * Bx;
-        F[5 * N + i] = mz * ux - Bz * Bx;
-        F[6 * N + i] = mx * uy - Bx * By;
-        F[7 * N + i] = my * uy - By * By + ptot;
-        F[8 * N + i] = mz * uy - Bz * By;
-        F[9 * N + i] = mx * uz - Bx * Bz;
-        F[10 * N + i] = my * uz - By * Bz;
-        F[11 * N + i] = mz * uz - Bz * Bz + ptot;
-        T ex = uz * By - uy * Bz, ey = ux * Bz - uz * Bx, ez = uy * Bx - ux * By;
-        if constexpr (HALL) {
-            const T Jx = R[8 * N + i], Jy = R[9 * N + i], Jz = R[10 * N + i];
-            const T h = di / rho;
-            ex = ex + h * (Jy * Bz - Jz * By);
-            ey = ey + h * (Jz * Bx - Jx * Bz);
-            ez = ez + h * (Jx * By - Jy * Bx);
-        }
-        F[12 * N + i] = ex;
-        F[13 * N + i] = ey;
-        F[14 * N + i] = ez;
-        const T ep = e + ptot;
-        F[15 * N + i] = ep * ux - udotb * Bx;
-        F[16 * N + i] = ep * uy - udotb * By;
-        F[17 * N + i] = ep * uz - udotb * Bz;
-        if constexpr (AEB) {
-            F[18 * N + i] = -(T(2) * g / (g - T(1)) * p / tau) - (T(2) * Bx * Bx + By * By + Bz * Bz) / tau -
-                            (mx * ux + T(2) * my * uy + T(2) * mz * uz) / tau;
-        }
-        if constexpr (CFL) {
-            const double d = cfl_point(double(rho), double(mx), double(my), double(mz), double(Bx), double(By),
-                                       double(Bz), double(p), double(ux), double(uy), double(uz), pp, rr, dmin);
-            dtloc = fmin(dtloc, d);
-        }
-    }
-    if constexpr (CFL) block_min_atomic(dtloc, &ctl->dtmin_bits);
-}
-
-template <typename T, bool FIRST, bool CFL, bool HALL>
-static void flux_aeb(bool aeb, const T* R, const double* IC, T* F, long long N, const PhysParams& pp, Ctl* ctl,
-                     hipStream_t s) {
-    const unsigned g = grid_for(N, kThreads, 8192);
-    if (aeb)
-        hipLaunchKernelGGL((k_flux<T, FIRST, CFL, HALL, true>), dim3(g), dim3(kThreads), 0, s, R, IC, F, N, pp, ctl);
-    else
-        hipLaunchKernelGGL((k_flux<T, FIRST, CFL, HALL, false>), dim3(g), dim3(kThreads), 0, s, R, IC, F, N, pp, ctl);
-}
-
-template <typename T, bool FIRST, bool CFL>
-static void flux_hall(bool hall, bool aeb, const T* R, const double* IC, T* F, long long N, const PhysParams& pp,
-                      Ctl* ctl, hipStream_t s) {
-    if (hall) flux_aeb<T, FIRST, CFL, true>(aeb, R, IC, F, N, pp, ctl, s);
-    else flux_aeb<T, FIRST, CFL, false>(aeb, R, IC, F, N, pp, ctl, s);
-}
-
-template <typename T>
-hipError_t launch_flux(const T* R, const double* IC, T* F, long long npts, int nflux, bool first, bool cfl,
-                       const PhysParams& pp, Ctl* ctl, hipStream_t s) {
-    (void)nflux;
-    const bool hall = pp.hall, aeb = pp.aeb;
-    if (first) {
-        if (cfl) flux_hall<T, true, true>(hall, aeb, R, IC, F, npts, pp, ctl, s);
-        else flux_hall<T, true, false>(hall, aeb, R, IC, F, npts, pp, ctl, s);
-    } else {
-        if (cfl) flux_hall<T, false, true>(hall, aeb, R, IC, F, npts, pp, ctl, s);
-        else flux_hall<T, false, false>(hall, aeb, R, IC, F, npts, pp, ctl, s);
-    }
-    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ RHS + RK3 + dealias
@@ -441,8 +301,8 @@ __global__ __launch_bounds__(kThreads) void k_cfl(const double* __restrict__ U, 
     if (pp.dz < dmin) dmin = pp.dz;
     double dtloc = __longlong_as_double(0x7FF0000000000000LL);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
-        const double d = cfl_point(U[i], U[N + i], U[2 * N + i], U[3 * N + i], U[4 * N + i], U[5 * N + i],
-                                   U[6 * N + i], P[3 * N + i], P[i], P[N + i], P[2 * N + i], pp, rr, dmin);
+        const double d = cfl_point(U[i], U[4 * N + i], U[5 * N + i], U[6 * N + i], P[3 * N + i], P[i], P[N + i],
+                                   P[2 * N + i], pp, rr, dmin);
         dtloc = fmin(dtloc, d);
     }
     block_min_atomic(dtloc, &ctl->dtmin_bits);
@@ -585,10 +445,6 @@ hipError_t launch_convert(const TI* in, TO* out, long long n, hipStream_t s) {
 template hipError_t launch_convert<double, float>(const double*, float*, long long, hipStream_t);
 template hipError_t launch_convert<float, double>(const float*, double*, long long, hipStream_t);
 
-template hipError_t launch_flux<double>(const double*, const double*, double*, long long, int, bool, bool,
-                                        const PhysParams&, Ctl*, hipStream_t);
-template hipError_t launch_flux<float>(const float*, const double*, float*, long long, int, bool, bool,
-                                       const PhysParams&, Ctl*, hipStream_t);
 template hipError_t launch_rhs_rk<double>(const cpx<double>*, cpx<double>*, cpx<double>*, Box, BoxTables, int,
                                           const PhysParams&, const Ctl*, hipStream_t);
 template hipError_t launch_rhs_rk<float>(const cpx<float>*, cpx<double>*, cpx<double>*, Box, BoxTables, int,
