@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run y = 2x over 1 GiB (1 GiB read + 1 GiB written) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -134,6 +136,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     n = args.n
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 2.0, out=y)
+        torch.cuda.synchronize()
+        log("calibration: 3 x elementwise y = 2x over 1 GiB (1 GiB read + 1 GiB write) done")
+        del x, y
     from laps_amd.params import LAPS_FP64, LAPS_MIXED
     log(f"building {n}^3 seeded initial condition")
     uu, pr = initial_state(n, 5.0 / 3.0)

@@ -42,9 +42,12 @@ struct Ctl {
 };
 
 // Kept-mode box of the spectral state: kx in [0,KX), ky/kz stored compactly.
+// Rows are stored with stride KXP >= KX (KXP a multiple of 16 for the kept
+// box, so every 128-byte segment of a row is one cache line); the entries
+// kx in [KX, KXP) are padding and stay zero.
 struct Box {
-    int KX, CY, CZ;
-    __host__ __device__ long long modes() const { return (long long)KX * CY * CZ; }
+    int KX, CY, CZ, KXP;
+    __host__ __device__ long long modes() const { return (long long)KXP * CY * CZ; }
 };
 
 // Per-box wave-number and dealias tables (device pointers).
@@ -85,15 +88,15 @@ hipError_t launch_x_r2c(int n, const T* in, cpx<T>* out, const XArgs& a, const c
 struct XFArgs {
     long long nrows;          // nz*ny rows per field
     long long npts;           // nx*ny*nz
-    const void* st;           // state spectral rows (T complex), kx_s per row
-    int kx_s;
+    const void* st;           // state spectral rows (T complex): kx_s stored of row stride rs_s
+    int kx_s, rs_s;
     long long fs_s;           // field stride of st (elements)
-    const void* jb;           // Hall current spectral rows, kx_j per row
-    int kx_j;
+    const void* jb;           // Hall current spectral rows
+    int kx_j, rs_j;
     long long fs_j;
     const double* S;          // first substage: uploaded uu(8) + uu_prim(4), FP64 real
-    void* out;                // flux spectral rows (T complex), kx_o per row
-    int kx_o;
+    void* out;                // flux spectral rows (T complex)
+    int kx_o, rs_o;
     long long fs_o;
 };
 template <typename T>

@@ -209,6 +209,7 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
     const int nx = c->nx, ny = c->ny, nz = c->nz, nxh = c->nxh;
     if (c->p.dealias_option != 1) {
         c->kept = c->full;
+        c->kept.KXP = (c->full.KX + 15) / 16 * 16;
         return LAPS_OK;
     }
     std::vector<double> ax(nxh), ay(ny), az(nz);
@@ -241,6 +242,7 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
     if (kxmax < 0) return fail(LAPS_EINVAL, "2/3 mask keeps no mode");
     Box b;
     b.KX = kxmax + 1;
+    b.KXP = (b.KX + 15) / 16 * 16;
     b.CY = std::min(2 * kymax + 1, ny);
     b.CZ = std::min(2 * kzmax + 1, nz);
     if (b.CY == ny - 1) b.CY = ny;  // keep compact rows symmetric or full
@@ -311,24 +313,25 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
                  cpx<T>* B) {
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
+    const long long RS = b.KXP;
     z.ncol = b.KX; z.in_cnt = b.CZ; z.out_cnt = (int)NZ;
-    z.in_ls = (long long)b.CY * b.KX; z.out_ls = (long long)b.CY * b.KX;
-    z.nb1 = b.CY; z.in_b1 = b.KX; z.out_b1 = b.KX;
-    z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * b.KX;
+    z.in_ls = (long long)b.CY * RS; z.out_ls = (long long)b.CY * RS;
+    z.nb1 = b.CY; z.in_b1 = RS; z.out_b1 = RS;
+    z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * RS;
     const double ez = (double)nf * b.CY * b.KX;
     LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + NZ * 2.0 * sizeof(T)),
                   [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream); }));
     StridedArgs y{};
     y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
-    y.in_ls = b.KX; y.out_ls = b.KX;
-    y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * b.KX; y.out_b1 = NY * b.KX;
-    y.nb2 = nf; y.in_b2 = NZ * b.CY * b.KX; y.out_b2 = NZ * NY * b.KX;
+    y.in_ls = RS; y.out_ls = RS;
+    y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * RS; y.out_b1 = NY * RS;
+    y.nb2 = nf; y.in_b2 = NZ * b.CY * RS; y.out_b2 = NZ * NY * RS;
     const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
     LAPS_CK(timed(c, K_YINV, ey * (b.CY + NY),
                   [&] { return launch_strided<T, T>(c->ny, +1, A, B, y, tw_of<T>(c, 1), T(1), c->stream); }));
     if (!dst) return LAPS_OK;
     XArgs x{};
-    x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = b.KX; x.rrow = NX;
+    x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = RS; x.rrow = NX;
     LAPS_CK(timed(c, K_XC2R, (double)x.nrows * sizeof(T) * (2.0 * b.KX + NX),
                   [&] { return launch_x_c2r<T>(c->nx, B, dst, x, tw_of<T>(c, 0), c->stream); }));
     return LAPS_OK;
@@ -340,25 +343,26 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
 template <typename T>
 int fwd_pipeline(laps_ctx* c, const T* src, int nf, const Box& b, cpx<T>* A, cpx<T>* B) {
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
+    const long long RS = b.KXP;
     if (src) {
         XArgs x{};
-        x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = b.KX; x.rrow = NX;
+        x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = RS; x.rrow = NX;
         LAPS_CK(timed(c, K_XR2C, (double)x.nrows * sizeof(T) * (2.0 * b.KX + NX),
                       [&] { return launch_x_r2c<T>(c->nx, src, A, x, tw_of<T>(c, 0), c->stream); }));
     }
     StridedArgs y{};
     y.ncol = b.KX; y.in_cnt = (int)NY; y.out_cnt = b.CY;
-    y.in_ls = b.KX; y.out_ls = b.KX;
-    y.nb1 = (int)NZ; y.in_b1 = NY * b.KX; y.out_b1 = (long long)b.CY * b.KX;
-    y.nb2 = nf; y.in_b2 = NZ * NY * b.KX; y.out_b2 = NZ * b.CY * b.KX;
+    y.in_ls = RS; y.out_ls = RS;
+    y.nb1 = (int)NZ; y.in_b1 = NY * RS; y.out_b1 = (long long)b.CY * RS;
+    y.nb2 = nf; y.in_b2 = NZ * NY * RS; y.out_b2 = NZ * b.CY * RS;
     const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
     LAPS_CK(timed(c, K_YFWD, ey * (NY + b.CY),
                   [&] { return launch_strided<T, T>(c->ny, -1, A, B, y, tw_of<T>(c, 1), T(1), c->stream); }));
     StridedArgs z{};
     z.ncol = b.KX; z.in_cnt = (int)NZ; z.out_cnt = b.CZ;
-    z.in_ls = (long long)b.CY * b.KX; z.out_ls = (long long)b.CY * b.KX;
-    z.nb1 = b.CY; z.in_b1 = b.KX; z.out_b1 = b.KX;
-    z.nb2 = nf; z.in_b2 = NZ * b.CY * b.KX; z.out_b2 = (long long)b.CZ * b.CY * b.KX;
+    z.in_ls = (long long)b.CY * RS; z.out_ls = (long long)b.CY * RS;
+    z.nb1 = b.CY; z.in_b1 = RS; z.out_b1 = RS;
+    z.nb2 = nf; z.in_b2 = NZ * b.CY * RS; z.out_b2 = (long long)b.CZ * b.CY * RS;
     const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
     const double ez = (double)nf * b.CY * b.KX * 2.0 * sizeof(T);
     LAPS_CK(timed(c, K_ZFWD, ez * (NZ + b.CZ),
@@ -384,7 +388,8 @@ int substage(laps_ctx* c, int irk) {
     xa.S = c->S;
     xa.out = P1;
     xa.kx_o = k.KX;
-    xa.fs_o = NZ * NY * k.KX;
+    xa.rs_o = k.KXP;
+    xa.fs_o = NZ * NY * k.KXP;
     double xin;
     if (first) {
         // first substage: the uploaded real fields are used as they are; J
@@ -397,7 +402,8 @@ int substage(laps_ctx* c, int irk) {
             if ((rc = inv_pipeline<T>(c, c->Ufull + 8 * c->M, c->M, c->full, 3, (T*)nullptr, P1, P2))) return rc;
             xa.jb = P2;
             xa.kx_j = c->nxh;
-            xa.fs_j = NZ * NY * c->nxh;
+            xa.rs_j = c->full.KXP;
+            xa.fs_j = NZ * NY * c->full.KXP;
         }
         xin = 96.0 * c->N + (hall ? 3.0 * xa.nrows * c->nxh * 2 * sizeof(T) : 0.0);
     } else {
@@ -408,9 +414,11 @@ int substage(laps_ctx* c, int irk) {
         if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P2))) return rc;
         xa.st = P2;
         xa.kx_s = k.KX;
-        xa.fs_s = NZ * NY * k.KX;
+        xa.rs_s = k.KXP;
+        xa.fs_s = NZ * NY * k.KXP;
         xa.jb = P2 + 8 * xa.fs_s;
         xa.kx_j = k.KX;
+        xa.rs_j = k.KXP;
         xa.fs_j = xa.fs_s;
         xin = (double)c->nf_in() * xa.nrows * k.KX * 2 * sizeof(T);
     }
@@ -601,7 +609,7 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     c->N = (long long)c->nx * c->ny * c->nz;
     c->M = (long long)c->nxh * c->ny * c->nz;
     c->mixed = p->precision == LAPS_MIXED;
-    c->full = Box{c->nxh, c->ny, c->nz};
+    c->full = Box{c->nxh, c->ny, c->nz, c->nxh};
     c->pp = make_phys(*p);
     int rc;
     auto bail = [&](int code) {
@@ -635,15 +643,18 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     const Box& k = c->kept;
     const int nfi = c->nf_in(), nfo = c->nf_out();
-    long long p1 = std::max({(long long)nfi * NZ * k.CY * k.KX, (long long)nfo * NZ * NY * k.KX,
-                             (long long)nfo * k.CZ * k.CY * k.KX, 3LL * NZ * NY * c->nxh});
-    long long p2 = std::max({(long long)nfi * NZ * NY * k.KX, (long long)nfo * NZ * k.CY * k.KX, 3LL * NZ * NY * c->nxh});
+    long long p1 = std::max({(long long)nfi * NZ * k.CY * k.KXP, (long long)nfo * NZ * NY * k.KXP,
+                             (long long)nfo * k.CZ * k.CY * k.KXP, 3LL * NZ * NY * c->nxh});
+    long long p2 = std::max({(long long)nfi * NZ * NY * k.KXP, (long long)nfo * NZ * k.CY * k.KXP,
+                             3LL * NZ * NY * c->nxh});
     const size_t ts = c->mixed ? sizeof(float) : sizeof(double);
+    // FP64 single-field scratch: a full spectrum, or one kept-box field inverted along z and y
+    const size_t xs = (size_t)std::max(c->M, NZ * NY * (long long)k.KXP);
     (void)NX;
     if ((rc = c->allocn(&c->ctl, 1)) || (rc = c->allocn(&c->U, 11 * KB)) || (rc = c->allocn(&c->Ufull, 11 * c->M)) ||
         (rc = c->allocn(&c->Fp, 8 * KB)) || (rc = c->allocn(&c->S, 12 * c->N)) ||
         (rc = c->alloc(&c->P1, p1 * 2 * ts)) || (rc = c->alloc(&c->P2, p2 * 2 * ts)) ||
-        (rc = c->allocn(&c->X1, c->M)) || (rc = c->allocn(&c->X2, c->M)) || (rc = c->allocn(&c->Xr, c->N)) ||
+        (rc = c->allocn(&c->X1, xs)) || (rc = c->allocn(&c->X2, xs)) || (rc = c->allocn(&c->Xr, c->N)) ||
         (rc = c->allocn(&c->red, (size_t)c->nblk_red * 16)) || (rc = c->allocn(&c->dave, 8)) ||
         (rc = c->allocn(&c->dbits, 1)))
         return bail(rc);
@@ -669,6 +680,8 @@ int laps_set_state(laps_ctx* c, const double* uu, const double* uu_prim) {
     LAPS_CK(hipMemcpy(c->S + 8 * c->N, uu_prim, 4 * c->N * sizeof(double), hipMemcpyHostToDevice));
     for (int f = 0; f < 8; ++f)
         if ((rc = fwd_fp64(c, c->S + f * c->N, c->Ufull + f * c->M))) return rc;
+    LAPS_CK(hipMemsetAsync(c->U, 0, 11 * c->kb() * sizeof(cpx<double>), c->stream));
+    LAPS_CK(hipMemsetAsync(c->Fp, 0, 8 * c->kb() * sizeof(cpx<double>), c->stream));
     LAPS_CK(launch_gather_box(c->Ufull, c->U, 8, c->M, c->kb(), c->nxh, c->ny, c->nz, c->kept, c->stream));
     if ((rc = reset_ctl(c, 0.0, 0.0, false))) return rc;
     LAPS_CK(hipStreamSynchronize(c->stream));
@@ -790,7 +803,7 @@ int laps_get_spectral(laps_ctx* c, double* uhat) {
                 const int iy = (b.CY == c->ny || cy <= (b.CY - 1) / 2) ? cy : c->ny - (b.CY - cy);
                 for (int kx = 0; kx < b.KX; ++kx)
                     o[f * c->M + ((long long)iz * c->ny + iy) * c->nxh + kx] =
-                        box[f * KB + ((long long)cz * b.CY + cy) * b.KX + kx];
+                        box[f * KB + ((long long)cz * b.CY + cy) * b.KXP + kx];
             }
         }
     return LAPS_OK;

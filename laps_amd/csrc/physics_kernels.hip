@@ -57,11 +57,13 @@ __device__ __forceinline__ double ksquare(double kx, double ky, double kz, const
     return kx * kx + a * a + b * b;
 }
 
-__device__ __forceinline__ void box_index(long long idx, const Box& b, int& kx, int& cy, int& cz) {
-    kx = (int)(idx % b.KX);
-    const long long r = idx / b.KX;
+// storage index -> (kx, stored ky row, stored kz row); false for row padding
+__device__ __forceinline__ bool box_index(long long idx, const Box& b, int& kx, int& cy, int& cz) {
+    kx = (int)(idx % b.KXP);
+    const long long r = idx / b.KXP;
     cy = (int)(r % b.CY);
     cz = (int)(r / b.CY);
+    return kx < b.KX;
 }
 
 // ------------------------------------------------------------------ curl
@@ -71,7 +73,7 @@ __global__ void k_curl(const cpx<double>* __restrict__ U, cpx<double>* __restric
     const double radius = ctl->radius;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         int kx, cy, cz;
-        box_index(i, b, kx, cy, cz);
+        if (!box_index(i, b, kx, cy, cz)) continue;
         const KVec k = kvec(t.wx[kx], t.wy[cy], t.wz[cz], pp, radius);
         const cpx<double> bx = U[4 * fs + i], by = U[5 * fs + i], bz = U[6 * fs + i];
         J[i] = ik(k.ky, bz) - ik(k.kz, by);
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void k_rhs_rk(const cpx<T>* __restrict__ 
     const double c = ctl->cc[irk], d = ctl->dd[irk], ts = ctl->ts[irk];
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         int kx, cy, cz;
-        box_index(i, b, kx, cy, cz);
+        if (!box_index(i, b, kx, cy, cz)) continue;
         const double wx = t.wx[kx], wy = t.wy[cy], wz = t.wz[cz];
         const KVec k = kvec(wx, wy, wz, pp, radius);
         auto Fv = [&](int f) -> cpx<double> {
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(kThreads) void k_divb(const cpx<double>* __restrict
     double m = 0.0;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         int kx, cy, cz;
-        box_index(i, b, kx, cy, cz);
+        if (!box_index(i, b, kx, cy, cz)) continue;
         const KVec k = kvec(t.wx[kx], t.wy[cy], t.wz[cz], pp, radius);
         cpx<double> d = ik(k.kx, U[4 * fs + i]) + ik(k.ky, U[5 * fs + i]);
         d = d + ik(k.kz, U[6 * fs + i]);
@@ -418,7 +420,7 @@ __global__ void k_gather_box(const cpx<double>* __restrict__ full, cpx<double>* 
         const int f = (int)(i / n);
         const long long m = i % n;
         int kx, cy, cz;
-        box_index(m, b, kx, cy, cz);
+        if (!box_index(m, b, kx, cy, cz)) continue;
         const int iy = (b.CY == ny || cy <= (b.CY - 1) / 2) ? cy : ny - (b.CY - cy);
         const int iz = (b.CZ == nz || cz <= (b.CZ - 1) / 2) ? cz : nz - (b.CZ - cz);
         box[f * bstride + m] = full[f * fstride + ((long long)iz * ny + iy) * nxh + kx];

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_xfused(XFArgs a, const cpx<T>* __restri
             ux[r] = T(S[8 * N3 + x]); uy[r] = T(S[9 * N3 + x]); uz[r] = T(S[10 * N3 + x]); p[r] = T(S[11 * N3 + x]);
         }
     } else {
-        const cpx<T>* st = (const cpx<T>*)a.st + row * a.kx_s;
+        const cpx<T>* st = (const cpx<T>*)a.st + row * a.rs_s;
         const long long fs = a.fs_s;
         c2r_pair<N, T>(st, st + fs, a.kx_s, line, tw, lane, rho, mx);
         c2r_pair<N, T>(st + 2 * fs, st + 3 * fs, a.kx_s, line, tw, lane, my, mz);
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_xfused(XFArgs a, const cpx<T>* __restri
     }
 
     if constexpr (HALL) {
-        const cpx<T>* jb = (const cpx<T>*)a.jb + row * a.kx_j;
+        const cpx<T>* jb = (const cpx<T>*)a.jb + row * a.rs_j;
         const long long fs = a.fs_j;
         c2r_pair<N, T>(jb, jb + fs, a.kx_j, line, tw, lane, jx, jy);
         T dummy[4];
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void k_xfused(XFArgs a, const cpx<T>* __restri
         }
     }
 
-    cpx<T>* out = (cpx<T>*)a.out + row * a.kx_o;
+    cpx<T>* out = (cpx<T>*)a.out + row * a.rs_o;
     const long long fo = a.fs_o;
     const int kxo = valid ? a.kx_o : 0;
     T fa[4], fb[4];
