@@ -1,0 +1,511 @@
+! mhd_host.f90 -- Fortran driver of the MI355X LAPS step.
+!
+! Mirrors the reference driver `program mhd` (src_compressible/mhd.f90:1-287):
+! the same namelist groups read in the same order from mhd.input
+! (mhd.f90:30-53), the background / perturbation hooks (mhdinit.f90:183-1036,
+! the cases listed below), the restart path (restart.f90:17-63), the output
+! files outNNN.dat, grid.dat, parallel_info.dat, rms.dat, EBM_info.dat and
+! log (mhdoutput.f90, mhdrms.f90:39-51, AEBmod.f90:75-85, mhd.f90:431-457),
+! and the main loop (mhd.f90:170-287). The time step itself runs on the GPU
+! through the C ABI (module laps_c): laps_step performs the dt < 1e-8 test,
+! evolve, time/radius update and the next vardt on the device.
+!
+! One GPU replaces the MPI decomposition: &prl is read and ignored, and
+! parallel_info.dat records one task. An optional group &laps read after
+! &Hall selects nstep (stop after that many steps; < 0: no limit), the
+! precision ('fp64' or 'mixed') and the HIP device.
+program mhd_host
+    use, intrinsic :: iso_c_binding
+    use laps_c
+    implicit none
+
+    integer, parameter :: dp = kind(1.0d0)
+    real(dp), parameter :: pi = 3.141592653589793_dp
+
+    ! &genr
+    real(dp) :: tmax = 0, dtout = 1, dtrms = 1
+    logical :: output_primitive = .true., if_restart = .false.
+    integer :: n_start = 0
+    ! &numerical
+    real(dp) :: cfl = 0.5_dp, afx = 0.495_dp, afy = 0.495_dp, afz = 0.495_dp
+    integer :: dealias_option = 2
+    logical :: if_resis_exp = .false., if_visc_exp = .false., if_conserve_background = .false.
+    ! &prl
+    integer :: ndim_parallel = 2
+    ! &grid
+    integer :: nx = 128, ny = 128, nz = 64
+    real(dp) :: Lx = 2 * pi, Ly = 2 * pi, Lz = 2 * pi
+    ! &field
+    integer :: ifield = 0
+    real(dp) :: B0 = 1.0_dp, a0 = 0.05_dp, shear_width = 0, press0 = 0, U_fast = 0, U_slow = 0
+    real(dp) :: n_fast = 0, n_slow = 0, Bx0 = 0, By0 = 0, Bz0 = 0, jet_width = 1.0_dp, Vjet = 0.3_dp
+    ! &pert
+    integer :: ipert = 0, nmode = 16, nmodex = 16, nmodey = 16, nmodez = 16, wave_number_jet = 1
+    real(dp) :: db0 = 0, dbx = 0, dbz = 0, in_out_ratio = 1.0_dp, dv0 = 0, drho0 = 0
+    real(dp) :: B0_SB = 0, dB_SB = 0, R0_SB = 0, R1_SB = 0, H_SB = 0, Rm_SB = 0, Rd_SB = 0
+    ! &phys
+    real(dp) :: adiabatic_index = 5.0_dp / 3.0_dp, resistivity = 0, viscosity = 0
+    logical :: if_resis = .false., if_visc = .false.
+    ! &AEB
+    logical :: if_AEB = .false., if_corotating = .false.
+    real(dp) :: radius0 = 30.0_dp, Ur0 = 0, corotating_angle = 0
+    ! &Hall
+    logical :: if_hall = .false.
+    real(dp) :: ion_inertial_length = 0
+    ! &laps (optional, this build only)
+    integer :: nstep = -1, device = 0
+    character(len=8) :: precision = 'fp64'
+
+    namelist /genr/ tmax, dtout, dtrms, output_primitive, if_restart, n_start
+    namelist /numerical/ cfl, dealias_option, afx, afy, afz, if_resis_exp, if_visc_exp, if_conserve_background
+    namelist /prl/ ndim_parallel
+    namelist /grid/ nx, ny, nz, Lx, Ly, Lz
+    namelist /phys/ adiabatic_index, if_resis, resistivity, if_visc, viscosity
+    namelist /field/ ifield, B0, a0, shear_width, press0, U_fast, U_slow, &
+        n_fast, n_slow, Bx0, By0, Bz0, jet_width, Vjet
+    namelist /pert/ ipert, db0, nmode, nmodex, nmodey, nmodez, dbx, dbz, in_out_ratio, &
+        dv0, drho0, wave_number_jet, B0_SB, dB_SB, R0_SB, R1_SB, H_SB, Rm_SB, Rd_SB
+    namelist /AEB/ if_AEB, radius0, Ur0, if_corotating, corotating_angle
+    namelist /Hall/ if_hall, ion_inertial_length
+    namelist /laps/ nstep, precision, device
+
+    real(dp), allocatable :: uu(:, :, :, :), uu_prim(:, :, :, :), xgrid(:), ygrid(:), zgrid(:)
+    type(laps_params) :: p
+    type(c_ptr) :: ctx
+    real(dp) :: time, dt, radius, dtmin_raw, tout, toutrms, tlog, dtlog, max_divb, t_restart
+    real(dp) :: clocktime, clocktime_output
+    real(dp), parameter :: delta_clocktime_output = 60 * 50
+    integer(c_int64_t) :: istep
+    integer(c_int32_t) :: status
+    integer :: iout, ios, i
+    integer(8) :: clock_start, clock_now, clock_rate, clock_max
+    integer(8), parameter :: dstep_checksave = 40
+
+    open(unit=4, status='old', file='mhd.input')
+    read(4, genr)
+    read(4, numerical)
+    read(4, prl)
+    read(4, grid)
+    read(4, field)
+    read(4, pert)
+    read(4, phys)
+    read(4, AEB)
+    read(4, Hall)
+    read(4, laps, iostat=ios)
+    close(unit=4)
+
+    call system_clock(clock_start, clock_rate, clock_max)
+    if (.not. if_AEB) Ur0 = 0.0_dp   ! mhd.f90:88-90
+    if (.not. if_corotating) corotating_angle = 0.0_dp   ! AEBmod.f90:28-30
+
+    allocate(uu(nx, ny, nz, 8), uu_prim(nx, ny, nz, 4), xgrid(nx), ygrid(ny), zgrid(nz))
+    do i = 1, nx
+        xgrid(i) = (i - 1) * (Lx / nx)
+    enddo
+    do i = 1, ny
+        ygrid(i) = (i - 1) * (Ly / ny)
+    enddo
+    do i = 1, nz
+        zgrid(i) = (i - 1) * (Lz / nz)
+    enddo
+
+    call laps_default_params(p)
+    p%nx = nx; p%ny = ny; p%nz = nz
+    p%Lx = Lx; p%Ly = Ly; p%Lz = Lz
+    p%adiabatic_index = adiabatic_index
+    p%cfl = cfl
+    p%dealias_option = dealias_option
+    p%if_resis = merge(1, 0, if_resis); p%if_resis_exp = merge(1, 0, if_resis_exp)
+    p%if_visc = merge(1, 0, if_visc); p%if_visc_exp = merge(1, 0, if_visc_exp)
+    p%if_conserve_background = merge(1, 0, if_conserve_background)
+    p%afx = afx; p%afy = afy; p%afz = afz
+    p%resistivity = resistivity; p%viscosity = viscosity
+    p%if_hall = merge(1, 0, if_hall); p%ion_inertial_length = ion_inertial_length
+    p%if_aeb = merge(1, 0, if_AEB); p%if_corotating = merge(1, 0, if_corotating)
+    p%radius0 = radius0; p%Ur0 = Ur0; p%corotating_angle = corotating_angle
+    p%precision = merge(LAPS_MIXED, LAPS_FP64, trim(precision) == 'mixed')
+    p%device = device
+    call laps_check(laps_create(p, ctx), 'laps_create')
+    write(*, '(3x,a)') 'laps_create finished......'
+
+    t_restart = 0
+    if (if_restart) then
+        call read_restart(n_start, t_restart)
+        write(*, '(3x,a)') 'read_restart finished......'
+    else
+        call background_fields_initialize
+        call perturbation_initialize
+        write(*, '(3x,a)') 'initial fields finished......'
+    endif
+    call initial_calc_conserve_variable
+    call laps_check(laps_set_state(ctx, uu, uu_prim), 'laps_set_state')
+    if (if_restart) call laps_check(laps_set_time(ctx, t_restart, 0.0_dp), 'laps_set_time')
+
+    call system_clock(clock_now)
+    write(*, '(1x,a,f15.2,a)') 'Initialization takes ', real(clock_now - clock_start, dp) / clock_rate, ' seconds.'
+    call system_clock(clock_start)
+
+    call laps_check(laps_cfl(ctx, dtmin_raw, dt), 'laps_cfl')
+    call write_initial
+    call output_grid
+    call output_parallel_info
+
+    dtlog = min(dtout, dtrms) / 10.0_dp
+    time = t_restart
+    iout = n_start
+    tout = time + dtout
+    toutrms = time + dtrms
+    tlog = time + dtlog
+    istep = 0
+
+    write(*, '(2x,a,2x,f10.4,2x,a,2x,1pe12.4)') 'OUTPUT at time:', time, ',dt = ', dt
+    call output_uu(iout, time)
+    iout = iout + 1
+    call output_rms_all(time)
+    clocktime_output = delta_clocktime_output
+
+    principal: do
+        if (time >= tout) then
+            write(*, '(2x,a,2x,f10.4,2x,a,2x,1pe12.4)') 'OUTPUT at time:', time, ',dt = ', dt
+            call output_uu(iout, time)
+            iout = iout + 1
+            tout = tout + dtout
+        endif
+        if (time >= toutrms) then
+            call output_rms_all(time)
+            toutrms = toutrms + dtrms
+        endif
+        if (istep > 0 .and. modulo(istep, dstep_checksave) == 0) then
+            call system_clock(clock_now)
+            clocktime = real(clock_now - clock_start, dp) / clock_rate
+            if (clocktime >= clocktime_output) then
+                write(*, '(3x,a,f15.2,2x,a,2x,f10.4)') 'OUTPUT for backup at real time (sec):', clocktime, &
+                    ', time = ', time
+                call output_uu(999, time)
+                clocktime_output = clocktime_output + delta_clocktime_output
+            endif
+        endif
+
+        ! dt < 1e-8 test, evolve, time += dt, evolve_radius, next vardt (on the device)
+        call laps_check(laps_step(ctx, 1), 'laps_step')
+        call laps_check(laps_get_scalars(ctx, time, dt, radius, istep, status), 'laps_get_scalars')
+        if (status == LAPS_DT_STOP) then
+            write(*, *) '***********************************'
+            write(*, *) 'STOP: dt  < 1.e-8    '
+            write(*, '(1x,a14,1pe12.4,a14,i6)') '  time = ', time, 'istep =', istep
+            write(*, '(1x,a14,1pe12.4)') 'dt =', dt
+            write(*, *) '***********************************'
+            write(*, '(2x,a,2x,f10.4,2x,a,2x,1pe12.4)') 'OUTPUT at time:', time, ',dt = ', dt
+            call output_uu(iout, time)
+            call output_rms_all(time)
+            exit principal
+        endif
+
+        if (time >= tmax .or. (nstep >= 0 .and. istep >= nstep)) then
+            write(*, '(2x,a,2x,f10.4,2x,a,2x,1pe12.4)') 'OUTPUT at time:', time, ',dt = ', dt
+            call output_uu(iout, time)
+            call output_rms_all(time)
+            write(*, *) '-------------------------------------------------'
+            if (time >= tmax) then
+                write(*, *) 'Simulation ends because tmax is reached.'
+            else
+                write(*, *) 'Simulation ends because nstep is reached.'
+            endif
+            write(*, *) '-------------------------------------------------'
+            exit principal
+        endif
+
+        if (time >= tlog) then
+            call write_log
+            tlog = tlog + dtlog
+        endif
+    enddo principal
+
+    call laps_destroy(ctx)
+
+contains
+
+    ! background_fields_initialize (mhdinit.f90:183-299), ifield = 1..5
+    subroutine background_fields_initialize
+        real(dp), parameter :: T0 = 1.0_dp, rho0 = 1.0_dp, ninf = 1.0_dp
+        real(dp) :: zlow, zcent, zup, ylow, ycent, yup, r_yz, aa
+        integer :: iy, iz
+        uu = 0
+        uu(:, :, :, 1) = rho0
+        uu(:, :, :, 8) = rho0 * T0
+        select case (ifield)
+        case (1)
+            zlow = 0.25_dp * Lz; zup = 0.75_dp * Lz; zcent = 0.5_dp * Lz
+            do iz = 1, nz
+                if (zgrid(iz) < zcent) then
+                    uu(:, :, iz, 5) = B0 * tanh((zgrid(iz) - zlow) / a0)
+                    uu(:, :, iz, 1) = ninf + B0**2 / 2.0_dp / T0 / (cosh((zgrid(iz) - zlow) / a0))**2
+                else
+                    uu(:, :, iz, 5) = -B0 * tanh((zgrid(iz) - zup) / a0)
+                    uu(:, :, iz, 1) = ninf + B0**2 / 2.0_dp / T0 / (cosh((zgrid(iz) - zup) / a0))**2
+                endif
+            enddo
+            uu(:, :, :, 8) = uu(:, :, :, 1) * T0
+        case (2)
+            ylow = 0.25_dp * Ly; yup = 0.75_dp * Ly; ycent = 0.5_dp * Ly
+            aa = Ly * 0.5_dp * shear_width
+            do iy = 1, ny
+                if (ygrid(iy) < ycent) then
+                    uu(:, iy, :, 2) = 0.5_dp * (U_fast - U_slow) * tanh((ygrid(iy) - ylow) / aa) + 0.5_dp * (U_fast + U_slow)
+                    uu(:, iy, :, 1) = 0.5_dp * (n_fast - n_slow) * tanh((ygrid(iy) - ylow) / aa) + 0.5_dp * (n_fast + n_slow)
+                else
+                    uu(:, iy, :, 2) = -0.5_dp * (U_fast - U_slow) * tanh((ygrid(iy) - yup) / aa) + 0.5_dp * (U_fast + U_slow)
+                    uu(:, iy, :, 1) = -0.5_dp * (n_fast - n_slow) * tanh((ygrid(iy) - yup) / aa) + 0.5_dp * (n_fast + n_slow)
+                endif
+            enddo
+            uu(:, :, :, 8) = press0
+            uu(:, :, :, 5) = bx0
+            uu(:, :, :, 6) = by0
+            uu(:, :, :, 7) = bz0
+        case (3)
+            uu(:, :, :, 5) = bx0
+            uu(:, :, :, 6) = by0
+            uu(:, :, :, 7) = bz0
+            uu(:, :, :, 8) = press0
+        case (4)
+            ylow = 0.25_dp * Ly; yup = 0.75_dp * Ly; ycent = 0.5_dp * Ly
+            do iy = 1, ny
+                if (ygrid(iy) < ycent) then
+                    uu(:, iy, :, 5) = B0 * tanh((ygrid(iy) - ylow) / a0)
+                else
+                    uu(:, iy, :, 5) = -B0 * tanh((ygrid(iy) - yup) / a0)
+                endif
+                uu(:, iy, :, 8) = press0 - 0.5_dp * (uu(:, iy, :, 5))**2
+                uu(:, iy, :, 1) = ninf
+            enddo
+        case (5)
+            uu(:, :, :, 5) = bx0
+            uu(:, :, :, 6) = by0
+            uu(:, :, :, 7) = bz0
+            uu(:, :, :, 8) = press0
+            do iz = 1, nz
+                do iy = 1, ny
+                    r_yz = sqrt((zgrid(iz) - Lz / 2)**2 + (ygrid(iy) - Ly / 2)**2)
+                    uu(:, iy, iz, 2) = Vjet / 2 * (1 - tanh((r_yz - 1) / 0.1_dp))
+                enddo
+            enddo
+        case default
+            continue
+        end select
+    end subroutine background_fields_initialize
+
+    ! perturbation_initialize (mhdinit.f90:302-1036): ipert = 1 and 999. The
+    ! other hooks draw from the compiler's RNG; start those runs from an
+    ! out000.dat written by laps_amd.ic and if_restart = .true.
+    subroutine perturbation_initialize
+        real(dp) :: kx, s, c
+        integer :: ix
+        s = sin(corotating_angle)
+        c = cos(corotating_angle)
+        select case (ipert)
+        case (0)
+            continue
+        case (1)
+            kx = 2 * pi / Lx * wave_number_jet
+            do ix = 1, nx
+                uu(ix, :, :, 7) = uu(ix, :, :, 7) - db0 * sin(kx * xgrid(ix))
+                uu(ix, :, :, 4) = uu(ix, :, :, 4) + db0 / sqrt(uu(ix, :, :, 1)) * sin(kx * xgrid(ix))
+                uu(ix, :, :, 2) = uu(ix, :, :, 2) + db0 / sqrt(uu(ix, :, :, 1)) * cos(kx * xgrid(ix)) * s
+                uu(ix, :, :, 5) = uu(ix, :, :, 5) - db0 * cos(kx * xgrid(ix)) * s
+                uu(ix, :, :, 3) = uu(ix, :, :, 3) + db0 / sqrt(uu(ix, :, :, 1)) * cos(kx * xgrid(ix)) * c
+                uu(ix, :, :, 6) = uu(ix, :, :, 6) - db0 * cos(kx * xgrid(ix)) * c
+            enddo
+        case (999)
+            block
+                integer :: iy, iz, v
+                do iz = 1, nz
+                    do iy = 1, ny
+                        do v = 1, 8
+                            uu(:, iy, iz, v) = sqrt((ygrid(iy) - Ly / 2.0_dp)**2 + (zgrid(iz) - Lz / 2.0_dp)**2)
+                        enddo
+                    enddo
+                enddo
+            end block
+        case default
+            write(*, '(1x,a,i0,a)') 'ipert = ', ipert, ' is not built into this host: write out000.dat with ', &
+                'laps_amd.ic and restart from it (if_restart = .true., n_start = 0)'
+            error stop 3
+        end select
+    end subroutine perturbation_initialize
+
+    ! initial_calc_conserve_variable (mhdinit.f90:1038-1056)
+    subroutine initial_calc_conserve_variable
+        uu_prim(:, :, :, 1:3) = uu(:, :, :, 2:4)
+        uu_prim(:, :, :, 4) = uu(:, :, :, 8)
+        uu(:, :, :, 2) = uu(:, :, :, 1) * uu_prim(:, :, :, 1)
+        uu(:, :, :, 3) = uu(:, :, :, 1) * uu_prim(:, :, :, 2)
+        uu(:, :, :, 4) = uu(:, :, :, 1) * uu_prim(:, :, :, 3)
+        uu(:, :, :, 8) = uu_prim(:, :, :, 4) / (adiabatic_index - 1) + &
+            0.5_dp * (uu(:, :, :, 1) * ((uu_prim(:, :, :, 1))**2 + &
+            (uu_prim(:, :, :, 2))**2 + (uu_prim(:, :, :, 3))**2) + &
+            (uu(:, :, :, 5))**2 + (uu(:, :, :, 6))**2 + (uu(:, :, :, 7))**2)
+    end subroutine initial_calc_conserve_variable
+
+    character(len=10) function out_name(n)
+        integer, intent(in) :: n
+        write(out_name, '(a3,i3.3,a4)') 'out', n, '.dat'
+    end function out_name
+
+    ! output_uu (mhdoutput.f90:72-131): [int32 4][float32 t][int32 4] then the
+    ! raw (nx,ny,nz,8) FP64 array at byte 13, primitives unless output_primitive is false
+    subroutine output_uu(n, t)
+        integer, intent(in) :: n
+        real(dp), intent(in) :: t
+        real(dp), allocatable :: w(:, :, :, :), wp(:, :, :, :)
+        integer :: u
+        allocate(w(nx, ny, nz, 8), wp(nx, ny, nz, 4))
+        call laps_check(laps_get_state(ctx, w, wp), 'laps_get_state')
+        if (output_primitive) then
+            w(:, :, :, 2:4) = wp(:, :, :, 1:3)
+            w(:, :, :, 8) = wp(:, :, :, 4)
+        endif
+        open(newunit=u, file=out_name(n), access='stream', form='unformatted', status='replace', action='write')
+        write(u) 4_c_int32_t, real(t, 4), 4_c_int32_t
+        write(u) w
+        close(u)
+        deallocate(w, wp)
+    end subroutine output_uu
+
+    ! read_restart (restart.f90:17-63): primitives of out{n_start}.dat; time is float32
+    subroutine read_restart(n, t)
+        integer, intent(in) :: n
+        real(dp), intent(out) :: t
+        integer(c_int32_t) :: m1, m2
+        real(4) :: t4
+        integer :: u
+        open(newunit=u, file=out_name(n), access='stream', form='unformatted', status='old', action='read')
+        read(u) m1, t4, m2
+        read(u) uu
+        close(u)
+        t = real(t4, dp)
+        write(*, '(1x,a,f10.4)') 'Restart from time = ', t
+    end subroutine read_restart
+
+    subroutine fortran_record(u, vals)
+        integer, intent(in) :: u
+        real(4), intent(in) :: vals(:)
+        write(u) int(4 * size(vals), c_int32_t), vals, int(4 * size(vals), c_int32_t)
+    end subroutine fortran_record
+
+    ! output_grid (mhdoutput.f90:51-60)
+    subroutine output_grid
+        integer :: u
+        open(newunit=u, file='grid.dat', access='stream', form='unformatted', status='replace', action='write')
+        call fortran_record(u, [real(nx, 4), real(ny, 4), real(nz, 4)])
+        call fortran_record(u, [real(xgrid, 4), real(ygrid, 4), real(zgrid, 4)])
+        close(u)
+    end subroutine output_grid
+
+    ! output_parallel_info (mhdoutput.f90:62-69): one task on one GPU
+    subroutine output_parallel_info
+        integer :: u
+        open(newunit=u, file='parallel_info.dat', access='stream', form='unformatted', status='replace', &
+            action='write')
+        call fortran_record(u, [1.0, 1.0, 1.0, 8.0])
+        close(u)
+    end subroutine output_parallel_info
+
+    ! calc_max_divB + output_rms + output_AEB (mhd.f90:183-192; mhdrms.f90:39-51; AEBmod.f90:75-85)
+    subroutine output_rms_all(t)
+        real(dp), intent(in) :: t
+        real(dp) :: ave(8), var(8), ru2(3), tt, ddt, rr
+        integer(c_int64_t) :: ist
+        integer(c_int32_t) :: st
+        integer :: u
+        logical :: ex
+        call laps_check(laps_max_divb(ctx, max_divb), 'laps_max_divb')
+        write(*, '(6x,a,2x,f10.4,a,1pe10.2,a,1pe12.4)') 'OUTPUT RMS at time:', t, ', max(div B) = ', max_divb, &
+            ', dt = ', dt
+        call laps_check(laps_moments(ctx, ave, var, ru2), 'laps_moments')
+        inquire(file='rms.dat', exist=ex)
+        if (ex) then
+            open(newunit=u, file='rms.dat', status='old', action='write', position='append')
+        else
+            open(newunit=u, file='rms.dat', status='replace', action='write')
+        endif
+        write(u, '(f12.6,2x,19(1pe16.8))') t, ave, var, ru2
+        close(u)
+        call laps_check(laps_get_scalars(ctx, tt, ddt, rr, ist, st), 'laps_get_scalars')
+        inquire(file='EBM_info.dat', exist=ex)
+        if (ex) then
+            open(newunit=u, file='EBM_info.dat', status='old', action='write', position='append')
+        else
+            open(newunit=u, file='EBM_info.dat', status='replace', action='write')
+        endif
+        write(u, '(3(1pe16.8))') t, rr, Ur0
+        close(u)
+    end subroutine output_rms_all
+
+    ! write_log (mhd.f90:431-457)
+    subroutine write_log
+        real(dp) :: ct
+        integer :: hour, minute, second, u
+        call system_clock(clock_now)
+        ct = real(clock_now - clock_start, dp) / clock_rate
+        hour = floor(ct / 3600.0_dp)
+        minute = floor((ct / 3600.0_dp - hour) * 60)
+        second = floor(((ct / 3600.0_dp - hour) * 60 - minute) * 60)
+        open(newunit=u, file='log', status='replace', action='write')
+        write(u, '(3x,a,f8.4)') 'Simulation time:', time
+        write(u, *) 'dt:', dt
+        write(u, '(3x,a,f15.2)') 'Real time (sec):', ct
+        write(u, '(3x,a,i3,a,i3,a,i3,a)') 'Real time (hh,mm,ss):', hour, 'h', minute, 'm', second, 's'
+        write(u, '(3x,a,i8)') 'Iterations     :', istep
+        write(u, *) 'tasks: ', 1
+        close(u)
+    end subroutine write_log
+
+    ! write_initial (mhd.f90:459-519)
+    subroutine write_initial
+        write(*, *) '--------------------------------------------------'
+        write(*, '(5x,a8,f10.2)') 'cfl = ', cfl
+        write(*, '(5x,a8,1pe10.2)') 'dt = ', dt
+        write(*, '(3(3x,a6,i6))') 'nx = ', nx, 'ny = ', ny, 'nz = ', nz
+        write(*, '(3x,a,a)') 'device: one MI355X, precision = ', trim(precision)
+        if (output_primitive) then
+            write(*, *) 'Output primitive variables...'
+        else
+            write(*, *) 'Output conserved variables...'
+        endif
+        if (if_resis) then
+            if (if_resis_exp) then
+                write(*, '(1x,a,1pe10.2)') 'Resistivity: ON. Explicit method. eta = ', resistivity
+            else
+                write(*, '(1x,a,1pe10.2)') 'Resistivity: ON. Implicit method. eta = ', resistivity
+            endif
+        else
+            write(*, '(1x,a)') 'Resistivity: OFF.'
+        endif
+        if (if_visc) then
+            if (if_visc_exp) then
+                write(*, '(1x,a,1pe10.2)') 'Viscosity: ON. Explicit method. nu = ', viscosity
+            else
+                write(*, '(1x,a,1pe10.2)') 'Viscosity: ON. Implicit method. nu = ', viscosity
+            endif
+        else
+            write(*, '(1x,a)') 'Viscosity: OFF.'
+        endif
+        if (if_AEB) then
+            write(*, '(1x,a,1pe10.2,a,1pe10.2)') 'Expanding Box: ON. Ur0 = ', Ur0, ', R0 = ', radius0
+        else
+            write(*, '(1x,a)') 'Expanding Box: OFF.'
+        endif
+        if (if_hall) then
+            write(*, '(1x,a,1pe10.2)') 'Hall term: ON. di = ', ion_inertial_length
+        else
+            write(*, '(1x,a)') 'Hall term: OFF.'
+        endif
+        if (dealias_option == 1) then
+            write(*, '(1x,a)') 'Use circular-padding for dealiasing.'
+        else if (dealias_option == 2) then
+            write(*, '(1x,3(a,f10.4))') 'Use smoothing function for dealiasing. afx = ', afx, ', afy = ', afy, &
+                ', afz = ', afz
+        endif
+        write(*, *) '--------------------------------------------------'
+    end subroutine write_initial
+end program mhd_host

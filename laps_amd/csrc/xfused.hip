@@ -29,6 +29,7 @@ struct LanePad {
 template <int N>
 struct LaneFft {
     static constexpr int LR = N / 4;                     // lanes per row
+    static constexpr int NT = LR > 64 ? LR : 64;         // threads per block: one wave, or one row
     static constexpr bool HAS2 = (N == 32 || N == 128 || N == 512);
 };
 
@@ -45,9 +46,14 @@ __device__ __forceinline__ void r4_stages(cpx<T> (&v)[4], cpx<T>* line, const cp
     }
     const int k = lane % NS;
     if constexpr (NS > 1) {
+        // W^k from the table, W^2k and W^3k by multiplication (one LDS read per stage)
         constexpr int STEP = N / (NS * 4);
-#pragma unroll
-        for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], twiddle<DIR>(tw, r * k * STEP));
+        const cpx<T> w1 = twiddle<DIR>(tw, k * STEP);
+        const cpx<T> w2 = cmul(w1, w1);
+        const cpx<T> w3 = cmul(w2, w1);
+        v[1] = cmul(v[1], w1);
+        v[2] = cmul(v[2], w2);
+        v[3] = cmul(v[3], w3);
     }
     Dft<4, DIR, T>::run(v);
     if constexpr (NS * 4 < N) {
@@ -152,14 +158,15 @@ __device__ __forceinline__ void r2c_pair(const T (&fa)[4], const T (&fb)[4], cpx
 }
 
 template <typename T, int N, bool FIRST, bool CFL, bool HALL, bool AEB>
-__global__ __launch_bounds__(256) void k_xfused(XFArgs a, const cpx<T>* __restrict__ twg, PhysParams pp,
+__global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T>* __restrict__ twg, PhysParams pp,
                                                 Ctl* __restrict__ ctl) {
     constexpr int LR = LaneFft<N>::LR;
-    constexpr int RPB = 256 / LR;  // rows per block
+    constexpr int NT = LaneFft<N>::NT;
+    constexpr int RPB = NT / LR;  // rows per block
     constexpr int LS = LanePad<T>::line(N);
     __shared__ cpx<T> sh[RPB * LS];
     __shared__ cpx<T> stw[N];
-    for (int i = threadIdx.x; i < N; i += 256) stw[i] = twg[i];
+    for (int i = threadIdx.x; i < N; i += NT) stw[i] = twg[i];
     __syncthreads();
     const cpx<T>* tw = stw;
     const int lane = threadIdx.x % LR;
@@ -192,9 +199,10 @@ __global__ __launch_bounds__(256) void k_xfused(XFArgs a, const cpx<T>* __restri
         const T g1 = T(pp.gamma) - T(1);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            ux[r] = mx[r] / rho[r];
-            uy[r] = my[r] / rho[r];
-            uz[r] = mz[r] / rho[r];
+            const T ir = T(1) / rho[r];
+            ux[r] = mx[r] * ir;
+            uy[r] = my[r] * ir;
+            uz[r] = mz[r] * ir;
             p[r] = (e[r] - T(0.5) * (mx[r] * ux[r] + my[r] * uy[r] + mz[r] * uz[r] + bx[r] * bx[r] + by[r] * by[r] +
                                      bz[r] * bz[r])) * g1;
         }
@@ -288,12 +296,13 @@ __global__ __launch_bounds__(256) void k_xfused(XFArgs a, const cpx<T>* __restri
 
 template <typename T, int N, bool FIRST, bool CFL, bool HALL>
 static void xf_aeb(bool aeb, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp, Ctl* ctl, hipStream_t s) {
-    constexpr int RPB = 256 / LaneFft<N>::LR;
+    constexpr int NT = LaneFft<N>::NT;
+    constexpr int RPB = NT / LaneFft<N>::LR;
     const unsigned g = (unsigned)((a.nrows + RPB - 1) / RPB);
     if (aeb)
-        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, true>), dim3(g), dim3(256), 0, s, a, tw, pp, ctl);
+        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, true>), dim3(g), dim3(NT), 0, s, a, tw, pp, ctl);
     else
-        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, false>), dim3(g), dim3(256), 0, s, a, tw, pp, ctl);
+        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, false>), dim3(g), dim3(NT), 0, s, a, tw, pp, ctl);
 }
 
 template <typename T, int N, bool FIRST, bool CFL>
