@@ -121,6 +121,7 @@ struct Dft<16, DIR, T> {
 };
 
 // Stage plans: radices R0, R1[, R2]. TPL = work-items per line (max butterflies).
+// Plan: two stages where possible (one LDS exchange, 16 values per work-item at 256).
 template <int N>
 struct Plan;
 template <> struct Plan<16>  { static constexpr int NST = 2, R0 = 4,  R1 = 4,  R2 = 1; };
@@ -130,9 +131,20 @@ template <> struct Plan<128> { static constexpr int NST = 2, R0 = 16, R1 = 8,  R
 template <> struct Plan<256> { static constexpr int NST = 2, R0 = 16, R1 = 16, R2 = 1; };
 template <> struct Plan<512> { static constexpr int NST = 3, R0 = 8,  R1 = 8,  R2 = 8; };
 
+// PlanLow: radices <= 8, last stage radix 4 where possible: fewer values per
+// work-item (more work-items per line) for kernels that keep a transform
+// result live across a second transform.
 template <int N>
+struct PlanLow;
+template <> struct PlanLow<16>  { static constexpr int NST = 2, R0 = 4, R1 = 4, R2 = 1; };
+template <> struct PlanLow<32>  { static constexpr int NST = 2, R0 = 8, R1 = 4, R2 = 1; };
+template <> struct PlanLow<64>  { static constexpr int NST = 3, R0 = 4, R1 = 4, R2 = 4; };
+template <> struct PlanLow<128> { static constexpr int NST = 3, R0 = 8, R1 = 4, R2 = 4; };
+template <> struct PlanLow<256> { static constexpr int NST = 3, R0 = 8, R1 = 8, R2 = 4; };
+template <> struct PlanLow<512> { static constexpr int NST = 3, R0 = 8, R1 = 8, R2 = 8; };
+
+template <int N, class P = Plan<N>>
 struct PlanInfo {
-    using P = Plan<N>;
     static constexpr int B0 = N / P::R0;
     static constexpr int B1 = N / P::R1;
     static constexpr int B2 = P::NST == 3 ? N / P::R2 : 0;
@@ -171,11 +183,10 @@ __device__ __forceinline__ int stage_out(int j, int r) {
 // input position p and `lds(p)` is the LDS address of position p of this
 // work-item's line. On return, if j < BLAST, v[r] holds output position
 // j + r*BLAST (r < RLAST). Every work-item of the block must call this.
-template <int N, int DIR, typename T, class Load, class Lds>
+template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds>
 __device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds,
                                               cpx<T>* v) {
-    using P = Plan<N>;
-    using I = PlanInfo<N>;
+    using I = PlanInfo<N, P>;
     // stage 0 (span 1): HBM -> registers -> LDS
     if (j < I::B0) {
 #pragma unroll
@@ -217,11 +228,11 @@ __device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ 
 }
 
 // Full line transform; `store(p, v)` consumes output position p.
-template <int N, int DIR, typename T, class Load, class Lds, class Store>
+template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds, class Store>
 __device__ __forceinline__ void line_fft(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds, Store store) {
-    using I = PlanInfo<N>;
+    using I = PlanInfo<N, P>;
     cpx<T> v[I::RMAX];
-    if (line_fft_regs<N, DIR, T>(j, tw, load, lds, v)) {
+    if (line_fft_regs<N, DIR, T, P>(j, tw, load, lds, v)) {
 #pragma unroll
         for (int r = 0; r < I::RLAST; ++r) store(j + r * I::BLAST, v[r]);
     }

@@ -16,20 +16,20 @@
 
 namespace laps {
 
-template <typename T, int N>
+template <typename T, int N, class P = Plan<N>>
 struct StridedCfg {
     static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
-    static constexpr int TPL = PlanInfo<N>::TPL;
+    static constexpr int TPL = PlanInfo<N, P>::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;  // odd line stride: adjacent lines hit different banks
 };
 
-template <typename TI, typename T, int N, int DIR>
-__global__ __launch_bounds__((StridedCfg<T, N>::NT)) void k_strided(const cpx<TI>* __restrict__ in,
+template <typename TI, typename T, int N, int DIR, class P>
+__global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx<TI>* __restrict__ in,
                                                                   cpx<T>* __restrict__ out, StridedArgs a,
                                                                   const cpx<T>* __restrict__ tw, T scale,
                                                                   int ntile) {
-    using C = StridedCfg<T, N>;
+    using C = StridedCfg<T, N, P>;
     __shared__ cpx<T> sh[C::TX * C::LS];
     const int t = threadIdx.x;
     const int l = t % C::TX, j = t / C::TX;
@@ -56,12 +56,12 @@ __global__ __launch_bounds__((StridedCfg<T, N>::NT)) void k_strided(const cpx<TI
         const int s = pos_to_stored(p, N, out_cnt);
         if (valid && s >= 0) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
     };
-    line_fft<N, DIR, T>(j, tw, load, lds, store);
+    line_fft<N, DIR, T, P>(j, tw, load, lds, store);
 }
 
 template <typename T, int N>
 struct XCfg {
-    static constexpr int TPL = PlanInfo<N>::TPL;
+    static constexpr int TPL = PlanInfo<N>::TPL;  // default plan
     static constexpr int LP = TPL >= 128 ? 1 : 128 / TPL;  // line pairs per work-group
     static constexpr int NT = LP * TPL;
     static constexpr int PADSH = sizeof(T) == 8 ? 4 : 5;    // one pad slot per 256 bytes
@@ -157,25 +157,32 @@ __global__ __launch_bounds__((XCfg<T, N>::NT)) void k_x_r2c(const T* __restrict_
 
 // ---------------------------------------------------------------- launchers
 
-template <typename TI, typename T, int N, int DIR>
-static hipError_t do_strided(const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw, T scale,
-                             hipStream_t s) {
-    using C = StridedCfg<T, N>;
+template <typename TI, typename T, int N, int DIR, class P>
+static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw, T scale,
+                               hipStream_t s) {
+    using C = StridedCfg<T, N, P>;
     const int ntile = (a.ncol + C::TX - 1) / C::TX;
     const long long nblk = (long long)ntile * a.nb1 * a.nb2;
     if (nblk <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_strided<TI, T, N, DIR>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, scale,
+    hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, scale,
                        ntile);
     return hipGetLastError();
 }
 
+template <typename TI, typename T, int N, int DIR>
+static hipError_t do_strided(int plan, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw,
+                             T scale, hipStream_t s) {
+    if (plan == 1) return do_strided_p<TI, T, N, DIR, PlanLow<N>>(in, out, a, tw, scale, s);
+    return do_strided_p<TI, T, N, DIR, Plan<N>>(in, out, a, tw, scale, s);
+}
+
 template <typename TI, typename T>
 hipError_t launch_strided(int n, int dir, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw,
-                          T scale, hipStream_t s) {
+                          T scale, hipStream_t s, int plan) {
 #define LAPS_STRIDED_CASE(NN)                                                              \
     case NN:                                                                               \
-        return dir < 0 ? do_strided<TI, T, NN, -1>(in, out, a, tw, scale, s)               \
-                       : do_strided<TI, T, NN, +1>(in, out, a, tw, scale, s);
+        return dir < 0 ? do_strided<TI, T, NN, -1>(plan, in, out, a, tw, scale, s)         \
+                       : do_strided<TI, T, NN, +1>(plan, in, out, a, tw, scale, s);
     switch (n) {
         LAPS_STRIDED_CASE(16)
         LAPS_STRIDED_CASE(32)
@@ -229,11 +236,11 @@ hipError_t launch_x_r2c(int n, const T* in, cpx<T>* out, const XArgs& a, const c
 }
 
 template hipError_t launch_strided<double, double>(int, int, const cpx<double>*, cpx<double>*, const StridedArgs&,
-                                                   const cpx<double>*, double, hipStream_t);
+                                                   const cpx<double>*, double, hipStream_t, int);
 template hipError_t launch_strided<double, float>(int, int, const cpx<double>*, cpx<float>*, const StridedArgs&,
-                                                  const cpx<float>*, float, hipStream_t);
+                                                  const cpx<float>*, float, hipStream_t, int);
 template hipError_t launch_strided<float, float>(int, int, const cpx<float>*, cpx<float>*, const StridedArgs&,
-                                                 const cpx<float>*, float, hipStream_t);
+                                                 const cpx<float>*, float, hipStream_t, int);
 template hipError_t launch_x_c2r<double>(int, const cpx<double>*, double*, const XArgs&, const cpx<double>*,
                                          hipStream_t);
 template hipError_t launch_x_c2r<float>(int, const cpx<float>*, float*, const XArgs&, const cpx<float>*,

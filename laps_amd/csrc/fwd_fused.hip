@@ -1,0 +1,324 @@
+// fwd_fused.hip -- forward y and z passes of the fluxes with the spectral
+// divergence / curl folded in, and the RK3 update folded into the z pass.
+//
+// Reference: calc_rhs (mhdrhs.f90:174-279), rkt (rktmod.f90:34-62),
+// dealias (dealiasing.f90:70-113) and the y/z legs of
+// transform_flux_real_to_fourier (fftw.f90:136-180).
+//
+// fnl = -(kx F_a + ky F_b + kz F_c) (k = i*k') is linear and each derivative
+// factor depends only on modes the pass already knows: kx' and ky' on the y
+// pass output, kz' on the z pass output. So the y pass of a divergence group
+// (F_a, F_b, F_c) writes  G = -i kx' Y(F_a) - i ky' Y(F_b)  and  Y(F_c)
+// (three fields in, two out), and the z pass forms fnl = Z(G) - i kz' Z(Y F_c)
+// per mode, where the RK3 update, implicit diffusion and dealiasing follow in
+// registers. The induction group (E) writes Y(E_x), Y(E_y) and the partial
+// curls P5 = -i ky' Y(E_z), P6 = i kx' Y(E_z), P7 = i ky' Y(E_x) - i kx' Y(E_y).
+// With corotating AEB, kx' and ky' both depend on (kx, ky); both are known per
+// output mode of the y pass.
+#include "internal.hpp"
+#include "physics_dev.hpp"
+
+namespace laps {
+
+template <typename T, int N>
+struct FwdCfg {
+    using P = PlanLow<N>;
+    using I = PlanInfo<N, P>;
+    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
+    static constexpr int TPL = I::TPL;
+    static constexpr int NT = TX * TPL;
+    static constexpr int LS = N + 1;
+};
+
+// derivative factors k' (real; the spectral factor is i*k'), mhdrhs.f90:192-204
+__device__ __forceinline__ void kprime_xy(double wx, double wy, const PhysParams& pp, double radius, double& kx,
+                                          double& ky) {
+    kx = wx;
+    ky = (wy * pp.radius0) / radius;
+    if (pp.corot_k) {
+        kx = wx * pp.cos_c + wy * pp.sin_c;
+        ky = ((-wx * pp.sin_c + wy * pp.cos_c) * pp.radius0) / radius;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> mul_i(T k, cpx<T> z) { return {-k * z.y, k * z.x}; }  // (i k) z
+
+// ---------------------------------------------------------------- y pass
+template <typename T, int N>
+__global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>* __restrict__ in,
+                                                                 cpx<T>* __restrict__ out, YGArgs a,
+                                                                 const cpx<T>* __restrict__ tw,
+                                                                 const double* __restrict__ wxt,
+                                                                 const double* __restrict__ wyf, PhysParams pp,
+                                                                 const Ctl* __restrict__ ctl, int ntile, int nz) {
+    using C = FwdCfg<T, N>;
+    using I = typename C::I;
+    __shared__ cpx<T> sh[C::TX * C::LS];
+    const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
+    long long bid = blockIdx.x;
+    const int tile = (int)(bid % ntile);
+    bid /= ntile;
+    const int z = (int)(bid % nz);
+    const int gi = (int)(bid / nz);
+    const YGroup G = a.g[gi];
+    const int col = tile * C::TX + l;
+    const bool valid = col < a.ncol;
+    const int kx = valid ? col : 0;
+    const double radius = ctl->radius;
+    const double wx = wxt[kx];
+    cpx<T>* line = sh + l * C::LS;
+    auto lds = [&](int p) { return line + p; };
+    const cpx<T>* ib = in + (long long)z * N * a.rs + kx;
+    cpx<T>* ob = out + (long long)z * a.cy * a.rs + kx;
+
+    auto fft_field = [&](int f, cpx<T>* v) {
+        const cpx<T>* src = ib + f * a.in_fs;
+        auto load = [&](int p) -> cpx<T> { return valid ? src[p * a.rs] : cpx<T>{T(0), T(0)}; };
+        __syncthreads();  // the LDS line may still be read by the previous transform
+        return line_fft_regs<N, -1, T, typename C::P>(j, tw, load, lds, v);
+    };
+    auto store = [&](int f, int p, cpx<T> val) {
+        const int s = pos_to_stored(p, N, a.cy);
+        if (valid && s >= 0) ob[f * a.out_fs + s * a.rs] = val;
+    };
+
+    cpx<T> va[I::RMAX], vb[I::RMAX];
+    if (G.kind == 2) {
+        if (fft_field(G.in[0], va)) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) store(G.out[0], j + r * I::BLAST, va[r]);
+        }
+        return;
+    }
+    const bool has = fft_field(G.in[0], va);
+    if (G.kind == 1 && has) {
+#pragma unroll
+        for (int r = 0; r < I::RLAST; ++r) store(G.out[0], j + r * I::BLAST, va[r]);  // Y(E_x)
+    }
+    fft_field(G.in[1], vb);
+    if (has) {
+#pragma unroll
+        for (int r = 0; r < I::RLAST; ++r) {
+            const int p = j + r * I::BLAST;
+            double kxp, kyp;
+            kprime_xy(wx, wyf[p], pp, radius, kxp, kyp);
+            const T kx_ = T(kxp), ky_ = T(kyp);
+            if (G.kind == 0) {
+                const cpx<T> gsum = mul_i(kx_, va[r]) + mul_i(ky_, vb[r]);
+                store(G.out[0], p, cpx<T>{-gsum.x, -gsum.y});                  // -(i kx' Ya + i ky' Yb)
+            } else {
+                store(G.out[1], p, vb[r]);                                       // Y(E_y)
+                store(G.out[4], p, mul_i(ky_, va[r]) - mul_i(kx_, vb[r]));      // P7
+            }
+        }
+    }
+    fft_field(G.in[2], va);
+    if (has) {
+#pragma unroll
+        for (int r = 0; r < I::RLAST; ++r) {
+            const int p = j + r * I::BLAST;
+            if (G.kind == 0) {
+                store(G.out[1], p, va[r]);                                       // Y(F_c)
+            } else {
+                double kxp, kyp;
+                kprime_xy(wx, wyf[p], pp, radius, kxp, kyp);
+                const cpx<T> t5 = mul_i(T(kyp), va[r]);
+                store(G.out[2], p, cpx<T>{-t5.x, -t5.y});                        // P5 = -i ky' Y(E_z)
+                store(G.out[3], p, mul_i(T(kxp), va[r]));                        // P6 =  i kx' Y(E_z)
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- z pass + RK3
+// Output variable v is  sum_t  m_t * Z(D[f_t])  with m_t in {1, -i kz', +i kz'}.
+template <typename T, int N, int OPT>
+__global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* __restrict__ D,
+                                                               cpx<double>* __restrict__ U,
+                                                               cpx<double>* __restrict__ Fp, ZArgs a,
+                                                               const cpx<T>* __restrict__ tw, BoxTables tb,
+                                                               const double* __restrict__ wzf, PhysParams pp,
+                                                               const Ctl* __restrict__ ctl, int irk, int ntile,
+                                                               T scale) {
+    using C = FwdCfg<T, N>;
+    using I = typename C::I;
+    if (ctl->status != 0) return;  // uniform over the grid: the step was refused
+    __shared__ cpx<T> sh[C::TX * C::LS];
+    const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
+    const int tile = (int)(blockIdx.x % ntile);
+    const int cy = (int)(blockIdx.x / ntile);
+    const int col = tile * C::TX + l;
+    const bool valid = col < a.KX;
+    const int kx = valid ? col : 0;
+    const double radius = ctl->radius, tau = ctl->tau;
+    const double c = ctl->cc[irk], d = ctl->dd[irk], ts = ctl->ts[irk];
+    const double wx = tb.wx[kx], wy = tb.wy[cy];
+    const int lim = tb.kzlim[(long long)cy * a.KX + kx];
+    cpx<T>* line = sh + l * C::LS;
+    auto lds = [&](int p) { return line + p; };
+    const cpx<T>* ib = D + (long long)cy * a.rs + kx;
+    const long long zls = (long long)a.CY * a.rs;   // z line stride of D
+    const long long mode0 = (long long)cy * a.rs + kx;
+
+    for (int v = 0; v < 8; ++v) {
+        const ZVar V = a.v[v];
+        cpx<T> acc[I::RMAX], w[I::RMAX];
+#pragma unroll
+        for (int r = 0; r < I::RMAX; ++r) acc[r] = cpx<T>{T(0), T(0)};
+        bool has = false;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            if (t >= V.nterm) break;
+            const cpx<T>* src = ib + V.t[t].field * a.in_fs;
+            auto load = [&](int p) -> cpx<T> { return valid ? src[p * zls] : cpx<T>{T(0), T(0)}; };
+            __syncthreads();
+            has = line_fft_regs<N, -1, T, typename C::P>(j, tw, load, lds, w);
+            if (has) {
+                const int m = V.t[t].mult;
+#pragma unroll
+                for (int r = 0; r < I::RLAST; ++r) {
+                    if (m == 0) {
+                        acc[r] = acc[r] + w[r];
+                    } else {
+                        const int p = j + r * I::BLAST;
+                        const T kz = T((wzf[p] * pp.radius0) / radius);
+                        const cpx<T> q = mul_i(kz, w[r]);
+                        acc[r] = m > 0 ? acc[r] + q : acc[r] - q;
+                    }
+                }
+            }
+        }
+        // stage fnl of this column in the LDS line, then update the kept modes in
+        // a rolled loop: adjacent work-items take adjacent kx (coalesced U, Fp)
+        __syncthreads();
+        if (has) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) *lds(j + r * I::BLAST) = acc[r];
+        }
+        __syncthreads();
+        if (!valid) continue;
+#pragma unroll 1
+        for (int p = j; p < N; p += C::TPL) {
+            const int cz = pos_to_stored(p, N, a.CZ);
+            if (cz < 0) continue;
+            const long long mi = (long long)cz * a.CY * a.rs + mode0;
+            const double wz = wzf[p];
+            const cpx<T> fv = *lds(p);
+            cpx<double> fnl{double(fv.x) * double(scale), double(fv.y) * double(scale)};
+            const cpx<double> u = U[v * a.ufs + mi];
+            if (pp.aeb && v < 7) {
+                const double fac = (v == 2 || v == 3) ? 3.0 : (v >= 5 ? 1.0 : 2.0);
+                fnl = fnl - cpx<double>{(u.x * fac) / tau, (u.y * fac) / tau};
+            }
+            const double k2 = (pp.visc || pp.resis) ? ksquare(wx, wy, wz, pp, ctl, radius) : 0.0;
+            if (pp.visc && pp.visc_exp && v >= 1 && v <= 3)
+                fnl = fnl - cpx<double>{u.x * pp.nu * k2, u.y * pp.nu * k2};
+            if (pp.resis && pp.resis_exp && v >= 4 && v <= 6 && !(pp.conserve_bg && kx == 0 && p == 0))
+                fnl = fnl - cpx<double>{u.x * pp.eta * k2, u.y * pp.eta * k2};
+            cpx<double> un;
+            if (irk == 0) {
+                un = cpx<double>{c * fnl.x + 0.0, c * fnl.y + 0.0} + u;
+            } else {
+                const cpx<double> fp = Fp[v * a.ufs + mi];
+                un = cpx<double>{c * fnl.x + d * fp.x, c * fnl.y + d * fp.y} + u;
+            }
+            Fp[v * a.ufs + mi] = fnl;
+            if (pp.visc && !pp.visc_exp && v >= 1 && v <= 3) {
+                const double den = ts * k2 * pp.nu + 1.0;
+                un = {un.x / den, un.y / den};
+            }
+            if (pp.resis && !pp.resis_exp && v >= 4 && v <= 6) {
+                const double den = ts * k2 * pp.eta + 1.0;
+                un = {un.x / den, un.y / den};
+            }
+            if constexpr (OPT == 1) {
+                const int mz = p <= N / 2 ? p : p - N;
+                if (mz > lim || -mz > lim) un = {0.0, 0.0};
+            } else if constexpr (OPT == 2) {
+                const double fx = tb.fx[kx], fy = tb.fy[cy], fz = tb.fz[cz];
+                un = {((un.x * fx) * fy) * fz, ((un.y * fx) * fy) * fz};
+            }
+            U[v * a.ufs + mi] = un;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <typename T, int N>
+static hipError_t do_yg(const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw, const double* wxt,
+                        const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz, hipStream_t s) {
+    using C = FwdCfg<T, N>;
+    const int ntile = (a.ncol + C::TX - 1) / C::TX;
+    const long long nblk = (long long)ntile * nz * a.ngroups;
+    hipLaunchKernelGGL((k_yfwd_group<T, N>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, wxt, wyf, pp,
+                       ctl, ntile, nz);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,
+                             const double* wxt, const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz,
+                             hipStream_t s) {
+    switch (ny) {
+        case 16: return do_yg<T, 16>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+        case 32: return do_yg<T, 32>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+        case 64: return do_yg<T, 64>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+        case 128: return do_yg<T, 128>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+        case 256: return do_yg<T, 256>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+        case 512: return do_yg<T, 512>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, int N, int OPT>
+static hipError_t do_zr(const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a, const cpx<T>* tw,
+                        const BoxTables& tb, const double* wzf, const PhysParams& pp, const Ctl* ctl, int irk,
+                        T scale, hipStream_t s) {
+    using C = FwdCfg<T, N>;
+    const int ntile = (a.KX + C::TX - 1) / C::TX;
+    const long long nblk = (long long)ntile * a.CY;
+    hipLaunchKernelGGL((k_zfwd_rhs<T, N, OPT>), dim3((unsigned)nblk), dim3(C::NT), 0, s, D, U, Fp, a, tw, tb, wzf, pp,
+                       ctl, irk, ntile, scale);
+    return hipGetLastError();
+}
+
+template <typename T, int N>
+static hipError_t do_zr_opt(const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a, const cpx<T>* tw,
+                            const BoxTables& tb, const double* wzf, const PhysParams& pp, const Ctl* ctl, int irk,
+                            T scale, hipStream_t s) {
+    if (pp.dealias == 1) return do_zr<T, N, 1>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+    if (pp.dealias == 2) return do_zr<T, N, 2>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+    return do_zr<T, N, 0>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+}
+
+template <typename T>
+hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a,
+                           const cpx<T>* tw, const BoxTables& tb, const double* wzf, const PhysParams& pp,
+                           const Ctl* ctl, int irk, T scale, hipStream_t s) {
+    switch (nz) {
+        case 16: return do_zr_opt<T, 16>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+        case 32: return do_zr_opt<T, 32>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+        case 64: return do_zr_opt<T, 64>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+        case 128: return do_zr_opt<T, 128>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+        case 256: return do_zr_opt<T, 256>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+        case 512: return do_zr_opt<T, 512>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_yfwd_group<double>(int, const cpx<double>*, cpx<double>*, const YGArgs&,
+                                              const cpx<double>*, const double*, const double*, const PhysParams&,
+                                              const Ctl*, int, hipStream_t);
+template hipError_t launch_yfwd_group<float>(int, const cpx<float>*, cpx<float>*, const YGArgs&, const cpx<float>*,
+                                             const double*, const double*, const PhysParams&, const Ctl*, int,
+                                             hipStream_t);
+template hipError_t launch_zfwd_rhs<double>(int, const cpx<double>*, cpx<double>*, cpx<double>*, const ZArgs&,
+                                            const cpx<double>*, const BoxTables&, const double*, const PhysParams&,
+                                            const Ctl*, int, double, hipStream_t);
+template hipError_t launch_zfwd_rhs<float>(int, const cpx<float>*, cpx<double>*, cpx<double>*, const ZArgs&,
+                                           const cpx<float>*, const BoxTables&, const double*, const PhysParams&,
+                                           const Ctl*, int, float, hipStream_t);
+
+}  // namespace laps

@@ -76,9 +76,10 @@ struct Twiddles {
 };
 
 // --- FFT launchers (fft_kernels.hip) ---
+// plan 0: two stages where possible (Plan), 1: radices <= 8 (PlanLow)
 template <typename TI, typename T>
 hipError_t launch_strided(int n, int dir, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a,
-                          const cpx<T>* tw, T scale, hipStream_t s);
+                          const cpx<T>* tw, T scale, hipStream_t s, int plan = 0);
 template <typename T>
 hipError_t launch_x_c2r(int n, const cpx<T>* in, T* out, const XArgs& a, const cpx<T>* tw, hipStream_t s);
 template <typename T>
@@ -103,12 +104,47 @@ template <typename T>
 hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp,
                          Ctl* ctl, hipStream_t s);
 
+// --- forward y / z passes with the RHS and RK3 folded in (fwd_fused.hip) ---
+struct YGroup {
+    int kind;     // 0 divergence (3 in, 2 out), 1 induction (3 in, 5 out), 2 copy (1 in, 1 out)
+    int in[3];
+    int out[5];
+};
+struct YGArgs {
+    int ncol, cy;
+    long long rs;
+    long long in_fs;
+    long long out_fs;
+    int ngroups;
+    YGroup g[8];
+};
+struct ZTerm {
+    int field;
+    int mult;     // 0: 1, -1: -i kz', +1: +i kz'
+};
+struct ZVar {
+    int nterm;
+    ZTerm t[3];
+};
+struct ZArgs {
+    int KX, CY, CZ;
+    long long rs;
+    long long in_fs;
+    long long ufs;
+    ZVar v[8];
+};
+template <typename T>
+hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,
+                             const double* wxt, const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz,
+                             hipStream_t s);
+template <typename T>
+hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a,
+                           const cpx<T>* tw, const BoxTables& tb, const double* wzf, const PhysParams& pp,
+                           const Ctl* ctl, int irk, T scale, hipStream_t s);
+
 // --- physics launchers (physics_kernels.hip) ---
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fstride, Box b, BoxTables t,
                        const PhysParams& pp, const Ctl* ctl, hipStream_t s);
-template <typename T>
-hipError_t launch_rhs_rk(const cpx<T>* E, cpx<double>* U, cpx<double>* Fp, Box b, BoxTables t, int irk,
-                         const PhysParams& pp, const Ctl* ctl, hipStream_t s);
 hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s);
 hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s);
 hipError_t launch_ctl_end(Ctl* ctl, const PhysParams& pp, hipStream_t s);

@@ -99,10 +99,10 @@ struct DevTables {
 
 // Kernel classes of the step pipeline, timed with HIP events on the
 // context's stream when laps_timing_enable() is on.
-enum KClass { K_ZINV, K_YINV, K_XFUSED, K_YFWD, K_ZFWD, K_RHSRK, K_CURL, K_CTL, K_XC2R, K_XR2C, K_NCLASS };
+enum KClass { K_ZINV, K_YINV, K_XFUSED, K_YFWD, K_ZFWD, K_CURL, K_CTL, K_XC2R, K_XR2C, K_NCLASS };
 static const char* const kClassNames[K_NCLASS] = {
-    "fft_z_inverse", "fft_y_inverse", "x_c2r_flux_r2c", "fft_y_forward", "fft_z_forward",
-    "rhs_rk3_dealias", "hall_curl", "control", "fft_x_c2r", "fft_x_r2c"};
+    "fft_z_inverse", "fft_y_inverse", "x_c2r_flux_r2c", "fft_y_fwd_divcurl", "fft_z_fwd_rhs_rk3",
+    "hall_curl", "control", "fft_x_c2r", "fft_x_r2c"};
 
 struct Timing {
     bool on = false;
@@ -140,6 +140,11 @@ struct laps_ctx {
     unsigned long long* dbits = nullptr;
     std::vector<void*> allocs;
     bool have_state = false, fresh = false, materialized = false, stepped_since_set = false;
+    bool tuned = false;
+    int plan_zinv = 0, plan_yinv = 0;  // FFT plans of the inverse z / y passes, chosen by autotune()
+    YGArgs yg{}, yg_aeb{};
+    ZArgs zr{}, zr_aeb{};
+    double yfwd_bytes = 0, zfwd_bytes = 0;  // algorithmic elements x 2 (multiply by sizeof(T))
     double fixed_dt = 0.0;
     int nblk_red = 1024;
 
@@ -284,6 +289,14 @@ hipError_t timed(laps_ctx* c, int cls, double bytes, F launch) {
     return e;
 }
 
+// timing off while in scope (utility transforms, autotuning)
+struct TimingOff {
+    laps_ctx* c;
+    bool was;
+    explicit TimingOff(laps_ctx* cc) : c(cc), was(cc->timing.on) { c->timing.on = false; }
+    ~TimingOff() { c->timing.on = was; }
+};
+
 int collect_timing(laps_ctx* c) {
     LAPS_CK(hipStreamSynchronize(c->stream));
     for (auto& pe : c->timing.pending) {
@@ -310,7 +323,7 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 // With dst == nullptr the x leg is left out and B holds the y-inverted rows.
 template <typename T>
 int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
-                 cpx<T>* B) {
+                 cpx<T>* B, int plan_z = 0, int plan_y = 0) {
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
     const long long RS = b.KXP;
@@ -320,7 +333,7 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
     z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * RS;
     const double ez = (double)nf * b.CY * b.KX;
     LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + NZ * 2.0 * sizeof(T)),
-                  [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream); }));
+                  [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
     y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
     y.in_ls = RS; y.out_ls = RS;
@@ -328,7 +341,7 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
     y.nb2 = nf; y.in_b2 = NZ * b.CY * RS; y.out_b2 = NZ * NY * RS;
     const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
     LAPS_CK(timed(c, K_YINV, ey * (b.CY + NY),
-                  [&] { return launch_strided<T, T>(c->ny, +1, A, B, y, tw_of<T>(c, 1), T(1), c->stream); }));
+                  [&] { return launch_strided<T, T>(c->ny, +1, A, B, y, tw_of<T>(c, 1), T(1), c->stream, plan_y); }));
     if (!dst) return LAPS_OK;
     XArgs x{};
     x.nrows = (long long)nf * NZ * NY; x.kx = b.KX; x.crow = RS; x.rrow = NX;
@@ -411,7 +424,8 @@ int substage(laps_ctx* c, int irk) {
             LAPS_CK(timed(c, K_CURL, 96.0 * KB, [&] {
                 return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
             }));
-        if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P2))) return rc;
+        if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv, c->plan_yinv)))
+            return rc;
         xa.st = P2;
         xa.kx_s = k.KX;
         xa.rs_s = k.KXP;
@@ -429,11 +443,68 @@ int substage(laps_ctx* c, int irk) {
         return launch_xfused<T>(c->nx, first, cfl, xa, tw_of<T>(c, 0), c->pp, c->ctl, c->stream);
     }));
     if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_finalize(c->ctl, c->pp, c->stream); }));
-    if ((rc = fwd_pipeline<T>(c, (const T*)nullptr, c->nf_out(), k, P1, P2))) return rc;
-    const double rb = (double)KB * (c->nf_out() * 2.0 * sizeof(T) + 16.0 * (8 + 8 + 8 + (irk ? 8 : 0)));
-    LAPS_CK(timed(c, K_RHSRK, rb, [&] {
-        return launch_rhs_rk<T>(P1, c->U, c->Fp, k, c->tkept.view(), irk, c->pp, c->ctl, c->stream);
+    // y forward with the x/y derivative terms folded in, then z forward + RHS + RK3 + dealias
+    const bool aeb = c->p.if_aeb != 0;
+    const YGArgs& ya = aeb ? c->yg_aeb : c->yg;
+    LAPS_CK(timed(c, K_YFWD, c->yfwd_bytes * sizeof(T), [&] {
+        return launch_yfwd_group<T>(c->ny, P1, P2, ya, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
+                                    c->nz, c->stream);
     }));
+    const ZArgs& za = aeb ? c->zr_aeb : c->zr;
+    const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
+    const double zb = c->zfwd_bytes * sizeof(T) + 16.0 * KB * (24 + (irk ? 8 : 0));
+    LAPS_CK(timed(c, K_ZFWD, zb, [&] {
+        return launch_zfwd_rhs<T>(c->nz, P2, c->U, c->Fp, za, tw_of<T>(c, 2), c->tkept.view(), c->tfull.wz, c->pp,
+                                  c->ctl, irk, scale, c->stream);
+    }));
+    return LAPS_OK;
+}
+
+// Times the inverse z and y passes of the kept box with both FFT plans
+// (events on the context stream, on the real buffers) and keeps the faster.
+template <typename T>
+int autotune(laps_ctx* c) {
+    TimingOff off(c);
+    const Box& k = c->kept;
+    cpx<T>* P1 = (cpx<T>*)c->P1;
+    cpx<T>* P2 = (cpx<T>*)c->P2;
+    hipEvent_t e0, e1;
+    LAPS_CK(hipEventCreate(&e0));
+    LAPS_CK(hipEventCreate(&e1));
+    float best[2][2] = {{0, 0}, {0, 0}};  // [pass][plan]
+    for (int plan = 0; plan < 2; ++plan) {
+        for (int rep = 0; rep < 4; ++rep) {
+            // z+y with plan for the y part held at 0, then y alone with plan
+            LAPS_CK(hipEventRecord(e0, c->stream));
+            int rc = inv_pipeline<T>(c, c->U, c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, plan, 0);
+            if (rc) return rc;
+            LAPS_CK(hipEventRecord(e1, c->stream));
+            LAPS_CK(hipEventSynchronize(e1));
+            float ms_zy = 0, ms_y = 0;
+            LAPS_CK(hipEventElapsedTime(&ms_zy, e0, e1));
+            const long long NY = c->ny, NZ = c->nz, RS = k.KXP;
+            StridedArgs y{};
+            y.ncol = k.KX; y.in_cnt = k.CY; y.out_cnt = (int)NY;
+            y.in_ls = RS; y.out_ls = RS;
+            y.nb1 = (int)NZ; y.in_b1 = (long long)k.CY * RS; y.out_b1 = NY * RS;
+            y.nb2 = c->nf_in(); y.in_b2 = NZ * k.CY * RS; y.out_b2 = NZ * NY * RS;
+            LAPS_CK(hipEventRecord(e0, c->stream));
+            LAPS_CK(launch_strided<T, T>(c->ny, +1, P1, P2, y, tw_of<T>(c, 1), T(1), c->stream, plan));
+            LAPS_CK(hipEventRecord(e1, c->stream));
+            LAPS_CK(hipEventSynchronize(e1));
+            LAPS_CK(hipEventElapsedTime(&ms_y, e0, e1));
+            if (rep == 0) continue;  // warm-up
+            best[1][plan] = (rep == 1 || ms_y < best[1][plan]) ? ms_y : best[1][plan];
+            const float z = ms_zy;   // z (with this plan) + y (plan 0)
+            best[0][plan] = (rep == 1 || z < best[0][plan]) ? z : best[0][plan];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // plan 1 only when clearly faster (3 %), so repeated runs pick the same code
+    c->plan_zinv = best[0][1] < 0.97f * best[0][0] ? 1 : 0;
+    c->plan_yinv = best[1][1] < 0.97f * best[1][0] ? 1 : 0;
+    c->tuned = true;
     return LAPS_OK;
 }
 
@@ -449,13 +520,6 @@ int one_step(laps_ctx* c) {
 }
 
 // FP64 single-field utilities (set_state, materialisation, laps_fft_*)
-struct TimingOff {
-    laps_ctx* c;
-    bool was;
-    explicit TimingOff(laps_ctx* cc) : c(cc), was(cc->timing.on) { c->timing.on = false; }
-    ~TimingOff() { c->timing.on = was; }
-};
-
 int fwd_fp64(laps_ctx* c, const double* real, cpx<double>* out_full) {
     TimingOff off(c);
     int rc = fwd_pipeline<double>(c, real, 1, c->full, c->X1, c->X2);
@@ -495,6 +559,71 @@ int materialize(laps_ctx* c) {
     LAPS_CK(launch_prim<double>(c->S, c->S + 8 * c->N, c->N, c->p.adiabatic_index, c->stream));
     c->materialized = true;
     return LAPS_OK;
+}
+
+// Forward-pass plan. Flux order of calc_flux (mhdrhs.f90:39-46): 0-2 rho*u,
+// 3-11 momentum fluxes (x block, y block, z block), 12-14 E, 15-17 energy
+// flux, 18 AEB energy source. y outputs D: 0/1 mass, 2/3, 4/5, 6/7 momentum,
+// 8 P5, 9 Y(E_y), 10 P6, 11 Y(E_x), 12 P7, 13/14 energy, 15 AEB source.
+void setup_forward_groups(laps_ctx* c) {
+    const Box& k = c->kept;
+    const long long NY = c->ny, NZ = c->nz;
+    for (int withaeb = 0; withaeb < 2; ++withaeb) {
+        YGArgs y{};
+        y.ncol = k.KX;
+        y.cy = k.CY;
+        y.rs = k.KXP;
+        y.in_fs = NZ * NY * k.KXP;
+        y.out_fs = NZ * k.CY * k.KXP;
+        const int div_in[5][3] = {{0, 1, 2}, {3, 4, 5}, {6, 7, 8}, {9, 10, 11}, {15, 16, 17}};
+        const int div_out[5][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {13, 14}};
+        int g = 0;
+        for (int d = 0; d < 5; ++d, ++g) {
+            y.g[g].kind = 0;
+            for (int t = 0; t < 3; ++t) y.g[g].in[t] = div_in[d][t];
+            y.g[g].out[0] = div_out[d][0];
+            y.g[g].out[1] = div_out[d][1];
+        }
+        y.g[g].kind = 1;
+        y.g[g].in[0] = 12; y.g[g].in[1] = 13; y.g[g].in[2] = 14;
+        y.g[g].out[0] = 11; y.g[g].out[1] = 9; y.g[g].out[2] = 8; y.g[g].out[3] = 10; y.g[g].out[4] = 12;
+        ++g;
+        if (withaeb) {
+            y.g[g].kind = 2;
+            y.g[g].in[0] = 18;
+            y.g[g].out[0] = 15;
+            ++g;
+        }
+        y.ngroups = g;
+        ZArgs z{};
+        z.KX = k.KX; z.CY = k.CY; z.CZ = k.CZ;
+        z.rs = k.KXP;
+        z.in_fs = NZ * k.CY * k.KXP;
+        z.ufs = k.modes();
+        auto two = [&](int v, int f0, int f1, int m1) {
+            z.v[v].nterm = 2;
+            z.v[v].t[0] = ZTerm{f0, 0};
+            z.v[v].t[1] = ZTerm{f1, m1};
+        };
+        two(0, 0, 1, -1);
+        two(1, 2, 3, -1);
+        two(2, 4, 5, -1);
+        two(3, 6, 7, -1);
+        two(4, 8, 9, +1);
+        two(5, 10, 11, -1);
+        z.v[6].nterm = 1;
+        z.v[6].t[0] = ZTerm{12, 0};
+        two(7, 13, 14, -1);
+        if (withaeb) {
+            z.v[7].nterm = 3;
+            z.v[7].t[2] = ZTerm{15, 0};
+        }
+        if (withaeb) { c->yg_aeb = y; c->zr_aeb = z; } else { c->yg = y; c->zr = z; }
+    }
+    const int nd = c->p.if_aeb ? 16 : 15;
+    const double cols = (double)NZ * k.KX;
+    c->yfwd_bytes = 2.0 * cols * ((double)c->nf_out() * NY + (double)nd * k.CY);
+    c->zfwd_bytes = 2.0 * (double)k.CY * k.KX * nd * NZ;
 }
 
 PhysParams make_phys(const laps_params& p) {
@@ -642,6 +771,7 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     const long long KB = c->kb();
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     const Box& k = c->kept;
+    setup_forward_groups(c);
     const int nfi = c->nf_in(), nfo = c->nf_out();
     long long p1 = std::max({(long long)nfi * NZ * k.CY * k.KXP, (long long)nfo * NZ * NY * k.KXP,
                              (long long)nfo * k.CZ * k.CY * k.KXP, 3LL * NZ * NY * c->nxh});
@@ -719,6 +849,10 @@ int laps_step(laps_ctx* c, int nsteps) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
     if (nsteps < 0) return fail(LAPS_EINVAL, "nsteps < 0");
+    if (!c->tuned && nsteps > 0) {
+        rc = c->mixed ? autotune<float>(c) : autotune<double>(c);
+        if (rc) return rc;
+    }
     for (int s = 0; s < nsteps; ++s) {
         rc = c->mixed ? one_step<float>(c) : one_step<double>(c);
         if (rc) return rc;

@@ -1,9 +1,6 @@
 // physics_kernels.hip -- pointwise kernels of the LAPS step.
 //
 //  k_curl     : Hall current J_hat = ik x B_hat      (mhdrhs.f90:313-339)
-//  k_rhs_rk   : spectral divergence/curl, AEB damping, explicit nu/eta, the
-//               low-storage RK3 update with implicit diffusion, dealiasing
-//               (mhdrhs.f90:174-279; rktmod.f90:34-62; dealiasing.f90:70-113)
 //  k_ctl_*    : the scalar bookkeeping of vardt / evolve_radius on the device
 //               (mhd.f90:216, :246-248, :418-428; rktmod.f90:15-32;
 //               AEBmod.f90:46-73)
@@ -42,20 +39,6 @@ __device__ __forceinline__ KVec kvec(double wx, double wy, double wz, const Phys
 
 // (i k) * z for real k
 __device__ __forceinline__ cpx<double> ik(double k, cpx<double> z) { return {-k * z.y, k * z.x}; }
-
-// k_square: grid_initialize form until the first evolve_radius, then
-// update_ksquare (mhdinit.f90:114-122; AEBmod.f90:87-125)
-__device__ __forceinline__ double ksquare(double kx, double ky, double kz, const PhysParams& pp, const Ctl* ctl) {
-    if (!ctl->ksq_updated) return kx * kx + ky * ky + kz * kz;
-    const double R0 = pp.radius0, R = ctl->radius;
-    if (pp.corot_ksq) {
-        const double c = pp.cos_c, s = pp.sin_c, r0r = R0 / R;
-        const double a = s * R0 / R, b = c * R0 / R, z = kz * R0 / R;
-        return kx * kx * (c * c + a * a) + ky * ky * (s * s + b * b) + kx * ky * 2 * c * s * (1 - r0r * r0r) + z * z;
-    }
-    const double a = ky * R0 / R, b = kz * R0 / R;
-    return kx * kx + a * a + b * b;
-}
 
 // storage index -> (kx, stored ky row, stored kz row); false for row padding
 __device__ __forceinline__ bool box_index(long long idx, const Box& b, int& kx, int& cy, int& cz) {
@@ -102,119 +85,6 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* b
         for (int i = 1; i < (int)(blockDim.x / 64); ++i) m = fmin(m, red[i]);
         atomicMin(bits, (unsigned long long)__double_as_longlong(m));
     }
-}
-
-// ------------------------------------------------------------------ RHS + RK3 + dealias
-template <typename T, int OPT>
-__global__ __launch_bounds__(kThreads) void k_rhs_rk(const cpx<T>* __restrict__ E, cpx<double>* __restrict__ U,
-                                                     cpx<double>* __restrict__ Fp, Box b, BoxTables t, int irk,
-                                                     PhysParams pp, const Ctl* __restrict__ ctl) {
-    if (ctl->status != 0) return;
-    const long long n = b.modes();
-    const double radius = ctl->radius, tau = ctl->tau;
-    const double c = ctl->cc[irk], d = ctl->dd[irk], ts = ctl->ts[irk];
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        int kx, cy, cz;
-        if (!box_index(i, b, kx, cy, cz)) continue;
-        const double wx = t.wx[kx], wy = t.wy[cy], wz = t.wz[cz];
-        const KVec k = kvec(wx, wy, wz, pp, radius);
-        auto Fv = [&](int f) -> cpx<double> {
-            const cpx<T> v = E[f * n + i];
-            return {double(v.x), double(v.y)};
-        };
-        cpx<double> r[8];
-        {
-            cpx<double> s = ik(k.kx, Fv(0)) + ik(k.ky, Fv(1));
-            s = s + ik(k.kz, Fv(2));
-            r[0] = {-s.x, -s.y};
-            s = ik(k.kx, Fv(3)) + ik(k.ky, Fv(4));
-            s = s + ik(k.kz, Fv(5));
-            r[1] = {-s.x, -s.y};
-            s = ik(k.kx, Fv(6)) + ik(k.ky, Fv(7));
-            s = s + ik(k.kz, Fv(8));
-            r[2] = {-s.x, -s.y};
-            s = ik(k.kx, Fv(9)) + ik(k.ky, Fv(10));
-            s = s + ik(k.kz, Fv(11));
-            r[3] = {-s.x, -s.y};
-            const cpx<double> e1 = Fv(12), e2 = Fv(13), e3 = Fv(14);
-            r[4] = ik(k.kz, e2) - ik(k.ky, e3);
-            r[5] = ik(k.kx, e3) - ik(k.kz, e1);
-            r[6] = ik(k.ky, e1) - ik(k.kx, e2);
-            s = ik(k.kx, Fv(15)) + ik(k.ky, Fv(16));
-            s = s + ik(k.kz, Fv(17));
-            r[7] = {-s.x, -s.y};
-        }
-        cpx<double> u[8];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) u[v] = U[v * n + i];
-        if (pp.aeb) {
-            const double f[7] = {2.0, 2.0, 3.0, 3.0, 2.0, 1.0, 1.0};
-#pragma unroll
-            for (int v = 0; v < 7; ++v) r[v] = r[v] - cpx<double>{(u[v].x * f[v]) / tau, (u[v].y * f[v]) / tau};
-            r[7] = r[7] + Fv(18);
-        }
-        const double k2 = ksquare(wx, wy, wz, pp, ctl);
-        if (pp.visc && pp.visc_exp) {
-#pragma unroll
-            for (int v = 1; v <= 3; ++v) r[v] = r[v] - cpx<double>{u[v].x * pp.nu * k2, u[v].y * pp.nu * k2};
-        }
-        if (pp.resis && pp.resis_exp && !(pp.conserve_bg && kx == 0 && t.mz[cz] == 0)) {
-#pragma unroll
-            for (int v = 4; v <= 6; ++v) r[v] = r[v] - cpx<double>{u[v].x * pp.eta * k2, u[v].y * pp.eta * k2};
-        }
-        // rkt (rktmod.f90:40-61)
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            cpx<double> un;
-            if (irk == 0) {
-                // fnl_rk was zeroed by rkt_init (rktmod.f90:22) and d_1 = 0
-                un = cpx<double>{c * r[v].x + 0.0, c * r[v].y + 0.0} + u[v];
-            } else {
-                const cpx<double> fp = Fp[v * n + i];
-                un = cpx<double>{c * r[v].x + d * fp.x, c * r[v].y + d * fp.y} + u[v];
-            }
-            Fp[v * n + i] = r[v];
-            u[v] = un;
-        }
-        if (pp.visc && !pp.visc_exp) {
-            const double den = ts * k2 * pp.nu + 1.0;
-#pragma unroll
-            for (int v = 1; v <= 3; ++v) u[v] = {u[v].x / den, u[v].y / den};
-        }
-        if (pp.resis && !pp.resis_exp) {
-            const double den = ts * k2 * pp.eta + 1.0;
-#pragma unroll
-            for (int v = 4; v <= 6; ++v) u[v] = {u[v].x / den, u[v].y / den};
-        }
-        // dealias (dealiasing.f90:152-175)
-        if constexpr (OPT == 1) {
-            const int lim = t.kzlim[(long long)cy * b.KX + kx];
-            const int mz = t.mz[cz];
-            if (mz > lim || -mz > lim) {
-#pragma unroll
-                for (int v = 0; v < 8; ++v) u[v] = {0.0, 0.0};
-            }
-        } else if constexpr (OPT == 2) {
-            const double fx = t.fx[kx], fy = t.fy[cy], fz = t.fz[cz];
-#pragma unroll
-            for (int v = 0; v < 8; ++v) u[v] = {((u[v].x * fx) * fy) * fz, ((u[v].y * fx) * fy) * fz};
-        }
-#pragma unroll
-        for (int v = 0; v < 8; ++v) U[v * n + i] = u[v];
-    }
-}
-
-template <typename T>
-hipError_t launch_rhs_rk(const cpx<T>* E, cpx<double>* U, cpx<double>* Fp, Box b, BoxTables t, int irk,
-                         const PhysParams& pp, const Ctl* ctl, hipStream_t s) {
-    const unsigned g = grid_for(b.modes());
-    if (pp.dealias == 1)
-        hipLaunchKernelGGL((k_rhs_rk<T, 1>), dim3(g), dim3(kThreads), 0, s, E, U, Fp, b, t, irk, pp, ctl);
-    else if (pp.dealias == 2)
-        hipLaunchKernelGGL((k_rhs_rk<T, 2>), dim3(g), dim3(kThreads), 0, s, E, U, Fp, b, t, irk, pp, ctl);
-    else
-        hipLaunchKernelGGL((k_rhs_rk<T, 0>), dim3(g), dim3(kThreads), 0, s, E, U, Fp, b, t, irk, pp, ctl);
-    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ control block
@@ -447,9 +317,5 @@ hipError_t launch_convert(const TI* in, TO* out, long long n, hipStream_t s) {
 template hipError_t launch_convert<double, float>(const double*, float*, long long, hipStream_t);
 template hipError_t launch_convert<float, double>(const float*, double*, long long, hipStream_t);
 
-template hipError_t launch_rhs_rk<double>(const cpx<double>*, cpx<double>*, cpx<double>*, Box, BoxTables, int,
-                                          const PhysParams&, const Ctl*, hipStream_t);
-template hipError_t launch_rhs_rk<float>(const cpx<float>*, cpx<double>*, cpx<double>*, Box, BoxTables, int,
-                                         const PhysParams&, const Ctl*, hipStream_t);
 
 }  // namespace laps
