@@ -179,18 +179,51 @@ __device__ __forceinline__ int stage_out(int j, int r) {
     return (j / NS) * NS * R + (j % NS) + r * NS;
 }
 
+struct NoPre {
+    __device__ void operator()() const {}
+};
+
+// A table of N values copied from HBM into LDS by the NT work-items of a block
+// in two steps: fetch() issues the loads into registers, put() writes LDS. A
+// kernel calls fetch() first and put() after issuing its own first loads, so
+// the table's latency hides behind them; a barrier must follow put().
+template <typename V, int N, int NT>
+struct LdsTable {
+    static constexpr int PER = (N + NT - 1) / NT;
+    V r[PER];
+    __device__ __forceinline__ void fetch(const V* __restrict__ g) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int t = (int)threadIdx.x + i * NT;
+            r[i] = g[t < N ? t : N - 1];  // every element assigned: the array stays in registers
+        }
+    }
+    __device__ __forceinline__ void put(V* s) const {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int t = (int)threadIdx.x + i * NT;
+            if (t < N) s[t] = r[i];
+        }
+    }
+};
+
 // Full line transform up to the last stage's butterfly. `load(p)` supplies
 // input position p and `lds(p)` is the LDS address of position p of this
-// work-item's line. On return, if j < BLAST, v[r] holds output position
-// j + r*BLAST (r < RLAST). Every work-item of the block must call this.
-template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds>
+// work-item's line. `pre()` runs on every work-item after the first stage's
+// loads are issued and before the first barrier. On return, if j < BLAST, v[r]
+// holds output position j + r*BLAST (r < RLAST). Every work-item of the block
+// must call this. `tw` may point to LDS or HBM.
+template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds, class Pre = NoPre>
 __device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds,
-                                              cpx<T>* v) {
+                                              cpx<T>* v, Pre pre = {}) {
     using I = PlanInfo<N, P>;
     // stage 0 (span 1): HBM -> registers -> LDS
     if (j < I::B0) {
 #pragma unroll
         for (int r = 0; r < P::R0; ++r) v[r] = load(j + r * I::B0);
+    }
+    pre();
+    if (j < I::B0) {
         stage_bfly<N, P::R0, 1, DIR, T>(v, j, tw);
 #pragma unroll
         for (int r = 0; r < P::R0; ++r) *lds(stage_out<1, P::R0>(j, r)) = v[r];
@@ -228,11 +261,12 @@ __device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ 
 }
 
 // Full line transform; `store(p, v)` consumes output position p.
-template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds, class Store>
-__device__ __forceinline__ void line_fft(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds, Store store) {
+template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds, class Store, class Pre = NoPre>
+__device__ __forceinline__ void line_fft(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds, Store store,
+                                         Pre pre = {}) {
     using I = PlanInfo<N, P>;
     cpx<T> v[I::RMAX];
-    if (line_fft_regs<N, DIR, T, P>(j, tw, load, lds, v)) {
+    if (line_fft_regs<N, DIR, T, P>(j, tw, load, lds, v, pre)) {
 #pragma unroll
         for (int r = 0; r < I::RLAST; ++r) store(j + r * I::BLAST, v[r]);
     }

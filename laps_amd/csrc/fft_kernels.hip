@@ -31,6 +31,9 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
                                                                   int ntile) {
     using C = StridedCfg<T, N, P>;
     __shared__ cpx<T> sh[C::TX * C::LS];
+    __shared__ cpx<T> stw[N];
+    LdsTable<cpx<T>, N, C::NT> twt;
+    twt.fetch(tw);
     const int t = threadIdx.x;
     const int l = t % C::TX, j = t / C::TX;
     long long bid = blockIdx.x;
@@ -56,7 +59,8 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
         const int s = pos_to_stored(p, N, out_cnt);
         if (valid && s >= 0) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
     };
-    line_fft<N, DIR, T, P>(j, tw, load, lds, store);
+    cpx<T>* const stp = stw;
+    line_fft<N, DIR, T, P>(j, stw, load, lds, store, [twt, stp]() { twt.put(stp); });
 }
 
 template <typename T, int N>

@@ -55,6 +55,14 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
     using C = FwdCfg<T, N>;
     using I = typename C::I;
     __shared__ cpx<T> sh[C::TX * C::LS];
+    __shared__ cpx<T> stw[N];
+    __shared__ double swy[N];
+    LdsTable<cpx<T>, N, C::NT> twt;
+    LdsTable<double, N, C::NT> wyt;
+    twt.fetch(tw);
+    wyt.fetch(wyf);
+    twt.put(stw);  // visible after the barrier that opens each transform
+    wyt.put(swy);
     const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
     long long bid = blockIdx.x;
     const int tile = (int)(bid % ntile);
@@ -76,7 +84,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
         const cpx<T>* src = ib + f * a.in_fs;
         auto load = [&](int p) -> cpx<T> { return valid ? src[p * a.rs] : cpx<T>{T(0), T(0)}; };
         __syncthreads();  // the LDS line may still be read by the previous transform
-        return line_fft_regs<N, -1, T, typename C::P>(j, tw, load, lds, v);
+        return line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, v);
     };
     auto store = [&](int f, int p, cpx<T> val) {
         const int s = pos_to_stored(p, N, a.cy);
@@ -102,7 +110,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
         for (int r = 0; r < I::RLAST; ++r) {
             const int p = j + r * I::BLAST;
             double kxp, kyp;
-            kprime_xy(wx, wyf[p], pp, radius, kxp, kyp);
+            kprime_xy(wx, swy[p], pp, radius, kxp, kyp);
             const T kx_ = T(kxp), ky_ = T(kyp);
             if (G.kind == 0) {
                 const cpx<T> gsum = mul_i(kx_, va[r]) + mul_i(ky_, vb[r]);
@@ -122,7 +130,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
                 store(G.out[1], p, va[r]);                                       // Y(F_c)
             } else {
                 double kxp, kyp;
-                kprime_xy(wx, wyf[p], pp, radius, kxp, kyp);
+                kprime_xy(wx, swy[p], pp, radius, kxp, kyp);
                 const cpx<T> t5 = mul_i(T(kyp), va[r]);
                 store(G.out[2], p, cpx<T>{-t5.x, -t5.y});                        // P5 = -i ky' Y(E_z)
                 store(G.out[3], p, mul_i(T(kxp), va[r]));                        // P6 =  i kx' Y(E_z)
@@ -145,6 +153,14 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
     using I = typename C::I;
     if (ctl->status != 0) return;  // uniform over the grid: the step was refused
     __shared__ cpx<T> sh[C::TX * C::LS];
+    __shared__ cpx<T> stw[N];
+    __shared__ double swz[N];
+    LdsTable<cpx<T>, N, C::NT> twt;
+    LdsTable<double, N, C::NT> wzt;
+    twt.fetch(tw);
+    wzt.fetch(wzf);
+    twt.put(stw);  // visible after the barrier that opens each transform
+    wzt.put(swz);
     const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
     const int tile = (int)(blockIdx.x % ntile);
     const int cy = (int)(blockIdx.x / ntile);
@@ -173,7 +189,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
             const cpx<T>* src = ib + V.t[t].field * a.in_fs;
             auto load = [&](int p) -> cpx<T> { return valid ? src[p * zls] : cpx<T>{T(0), T(0)}; };
             __syncthreads();
-            has = line_fft_regs<N, -1, T, typename C::P>(j, tw, load, lds, w);
+            has = line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, w);
             if (has) {
                 const int m = V.t[t].mult;
 #pragma unroll
@@ -182,7 +198,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
                         acc[r] = acc[r] + w[r];
                     } else {
                         const int p = j + r * I::BLAST;
-                        const T kz = T((wzf[p] * pp.radius0) / radius);
+                        const T kz = T((swz[p] * pp.radius0) / radius);
                         const cpx<T> q = mul_i(kz, w[r]);
                         acc[r] = m > 0 ? acc[r] + q : acc[r] - q;
                     }
@@ -203,7 +219,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
             const int cz = pos_to_stored(p, N, a.CZ);
             if (cz < 0) continue;
             const long long mi = (long long)cz * a.CY * a.rs + mode0;
-            const double wz = wzf[p];
+            const double wz = swz[p];
             const cpx<T> fv = *lds(p);
             cpx<double> fnl{double(fv.x) * double(scale), double(fv.y) * double(scale)};
             const cpx<double> u = U[v * a.ufs + mi];

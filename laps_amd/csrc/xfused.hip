@@ -19,11 +19,18 @@
 
 namespace laps {
 
+// LDS slot of line position p: an XOR swizzle inside each 128-byte group
+// (M + 1 complex values). Lanes are served in groups that together cover
+// 128 bytes (8 lanes of 16-byte values, 16 lanes of 8-byte values); with
+// the swizzle, the contiguous reads (lane + r*LR) and the strided stage
+// writes ((lane/NS)*4NS + lane%NS + r*NS, NS = 1, 2, 4, 16, ...) of one
+// group land in distinct banks. Only the mirrored read of the R2C split
+// pairs up banks. No padding, so a line holds exactly N values.
 template <typename T>
 struct LanePad {
-    static constexpr int SH = sizeof(T) == 8 ? 4 : 5;  // one pad slot per 256 bytes
-    __device__ static constexpr int at(int p) { return p + (p >> SH); }
-    static constexpr int line(int n) { return n + (n >> SH) + 1; }
+    static constexpr int M = 128 / (2 * (int)sizeof(T)) - 1;
+    __device__ static constexpr int at(int p) { return p ^ ((p >> 2) & M); }
+    static constexpr int line(int n) { return n; }
 };
 
 template <int N>
@@ -33,16 +40,21 @@ struct LaneFft {
     static constexpr bool HAS2 = (N == 32 || N == 128 || N == 512);
 };
 
-// radix-4 Stockham stages of span NS, NS*4, ... up to N; v[r] is position
-// lane + r*LR when read from the line and after the last stage.
-template <int N, int NS, int DIR, typename T>
-__device__ __forceinline__ void r4_stages(cpx<T> (&v)[4], cpx<T>* line, const cpx<T>* __restrict__ tw, int lane,
-                                          bool from_lds) {
+// radix-4 Stockham stages of span NS, NS*4, ... up to N, applied to K
+// independent transforms at once (each with its own LDS line), so that one
+// transform's LDS round trip overlaps the other's arithmetic. v[t][r] is
+// position lane + r*LR of transform t when read from the line and after the
+// last stage.
+template <int N, int NS, int DIR, int K, typename T>
+__device__ __forceinline__ void r4_stages(cpx<T> (&v)[K][4], cpx<T>* const (&line)[K], const cpx<T>* __restrict__ tw,
+                                          int lane, bool from_lds) {
     using P = LanePad<T>;
     constexpr int LR = LaneFft<N>::LR;
     if (from_lds) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = *(line + P::at(lane + r * LR));
+        for (int t = 0; t < K; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[t][r] = *(line[t] + P::at(lane + r * LR));
     }
     const int k = lane % NS;
     if constexpr (NS > 1) {
@@ -51,110 +63,173 @@ __device__ __forceinline__ void r4_stages(cpx<T> (&v)[4], cpx<T>* line, const cp
         const cpx<T> w1 = twiddle<DIR>(tw, k * STEP);
         const cpx<T> w2 = cmul(w1, w1);
         const cpx<T> w3 = cmul(w2, w1);
-        v[1] = cmul(v[1], w1);
-        v[2] = cmul(v[2], w2);
-        v[3] = cmul(v[3], w3);
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+            v[t][1] = cmul(v[t][1], w1);
+            v[t][2] = cmul(v[t][2], w2);
+            v[t][3] = cmul(v[t][3], w3);
+        }
     }
-    Dft<4, DIR, T>::run(v);
+#pragma unroll
+    for (int t = 0; t < K; ++t) Dft<4, DIR, T>::run(v[t]);
     if constexpr (NS * 4 < N) {
         __syncthreads();
         const int base = (lane / NS) * NS * 4 + k;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) *(line + P::at(base + r * NS)) = v[r];
+        for (int t = 0; t < K; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) *(line[t] + P::at(base + r * NS)) = v[t][r];
         __syncthreads();
-        r4_stages<N, NS * 4, DIR, T>(v, line, tw, lane, true);
+        r4_stages<N, NS * 4, DIR, K, T>(v, line, tw, lane, true);
     }
 }
 
-// In-place transform of one row: v[r] holds position lane + r*LR on entry and
-// on exit. `line` is this row's LDS line. All threads of the block call it.
-template <int N, int DIR, typename T>
-__device__ __forceinline__ void lane_fft(cpx<T> (&v)[4], cpx<T>* line, const cpx<T>* __restrict__ tw, int lane) {
+// In-place transform of K rows of data: v[t][r] holds position lane + r*LR
+// on entry and on exit. line[t] is transform t's LDS line. All threads of the
+// block call it.
+template <int N, int DIR, int K, typename T>
+__device__ __forceinline__ void lane_fft(cpx<T> (&v)[K][4], cpx<T>* const (&line)[K], const cpx<T>* __restrict__ tw,
+                                         int lane) {
     using P = LanePad<T>;
     constexpr int LR = LaneFft<N>::LR;
     if constexpr (LaneFft<N>::HAS2) {
         // radix-2 stage of span 1: butterfly j = lane pairs r = 0,2; j = lane + LR pairs r = 1,3;
         // outputs of butterfly j go to 2j and 2j + 1
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const cpx<T> a = v[h], b = v[h + 2];
-            v[h] = a + b;
-            v[h + 2] = a - b;
+        for (int t = 0; t < K; ++t)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const cpx<T> a = v[t][h], b = v[t][h + 2];
+                v[t][h] = a + b;
+                v[t][h + 2] = a - b;
+            }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+            *(line[t] + P::at(2 * lane)) = v[t][0];
+            *(line[t] + P::at(2 * lane + 1)) = v[t][2];
+            *(line[t] + P::at(2 * (lane + LR))) = v[t][1];
+            *(line[t] + P::at(2 * (lane + LR) + 1)) = v[t][3];
         }
         __syncthreads();
-        *(line + P::at(2 * lane)) = v[0];
-        *(line + P::at(2 * lane + 1)) = v[2];
-        *(line + P::at(2 * (lane + LR))) = v[1];
-        *(line + P::at(2 * (lane + LR) + 1)) = v[3];
-        __syncthreads();
-        r4_stages<N, 2, DIR, T>(v, line, tw, lane, true);
+        r4_stages<N, 2, DIR, K, T>(v, line, tw, lane, true);
     } else {
-        r4_stages<N, 1, DIR, T>(v, line, tw, lane, false);
+        r4_stages<N, 1, DIR, K, T>(v, line, tw, lane, false);
     }
 }
 
-// Z_p of the paired C2R (Z = A + iB with Hermitian extension; only the real
-// parts of the DC and Nyquist coefficients enter, as in FFTW's c2r).
+// Z_p, p = lane + r*LR, of the paired C2R (Z = A + iB with Hermitian
+// extension; only the real parts of the DC and Nyquist coefficients enter, as
+// in FFTW's c2r). Every lane loads A and B at the clamped index min(m, 0 if
+// m >= kx) of the mode m = min(p, N - p) it needs, with no branch around the
+// loads, so the eight loads of a pair are all in flight before the first use.
 template <int N, typename T>
-__device__ __forceinline__ cpx<T> c2r_input(const cpx<T>* __restrict__ A, const cpx<T>* __restrict__ B, int kx,
-                                            int p) {
-    const cpx<T> zero{T(0), T(0)};
-    if (p == 0) return cpx<T>{A[0].x, B ? B[0].x : T(0)};
-    if (p < N / 2) {
-        if (p >= kx) return zero;
-        const cpx<T> a = A[p], b = B ? B[p] : zero;
-        return cpx<T>{a.x - b.y, a.y + b.x};
-    }
-    if (p == N / 2) {
-        if (p >= kx) return zero;
-        return cpx<T>{A[p].x, B ? B[p].x : T(0)};
-    }
-    const int m = N - p;
-    if (m >= kx) return zero;
-    const cpx<T> a = A[m], b = B ? B[m] : zero;
-    return cpx<T>{a.x + b.y, b.x - a.y};
-}
-
-template <int N, typename T>
-__device__ __forceinline__ void c2r_pair(const cpx<T>* A, const cpx<T>* B, int kx, cpx<T>* line,
-                                         const cpx<T>* tw, int lane, T (&a)[4], T (&b)[4]) {
+__device__ __forceinline__ void c2r_inputs(const cpx<T>* __restrict__ A, const cpx<T>* __restrict__ B, int kx,
+                                           int lane, cpx<T> (&v)[4]) {
     constexpr int LR = LaneFft<N>::LR;
-    cpx<T> v[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = c2r_input<N, T>(A, B, kx, lane + r * LR);
-    lane_fft<N, +1, T>(v, line, tw, lane);
+    const cpx<T> zero{T(0), T(0)};
+    cpx<T> a[4], b[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        a[r] = v[r].x;
-        b[r] = v[r].y;
+        const int p = lane + r * LR;
+        const int m = r < 2 ? p : N - p;  // r < 2: p < N/2; r >= 2: p >= N/2
+        const int mc = m < kx ? m : 0;
+        a[r] = A[mc];
+        b[r] = B ? B[mc] : zero;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int p = lane + r * LR;
+        const int m = r < 2 ? p : N - p;
+        cpx<T> z;
+        if (p == 0 || p == N / 2) z = cpx<T>{a[r].x, b[r].x};
+        else if (r < 2) z = cpx<T>{a[r].x - b[r].y, a[r].y + b[r].x};
+        else z = cpx<T>{a[r].x + b[r].y, b[r].x - a[r].y};
+        v[r] = m < kx ? z : zero;
     }
 }
 
-// forward transform of Z = fa + i fb and the split into the two half spectra
+// Two paired C2Rs at once: spectra (A0, B0) to real rows (a0, b0) and
+// (A1, B1) to (a1, b1).
 template <int N, typename T>
-__device__ __forceinline__ void r2c_pair(const T (&fa)[4], const T (&fb)[4], cpx<T>* oa, cpx<T>* ob, int kx,
-                                         cpx<T>* line, const cpx<T>* tw, int lane) {
+__device__ __forceinline__ void c2r_pair2(const cpx<T>* A0, const cpx<T>* B0, const cpx<T>* A1, const cpx<T>* B1,
+                                          int kx, cpx<T>* const (&line)[2], const cpx<T>* tw, int lane, T (&a0)[4],
+                                          T (&b0)[4], T (&a1)[4], T (&b1)[4]) {
+    cpx<T> v[2][4];
+    c2r_inputs<N, T>(A0, B0, kx, lane, v[0]);
+    c2r_inputs<N, T>(A1, B1, kx, lane, v[1]);
+    lane_fft<N, +1, 2, T>(v, line, tw, lane);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        a0[r] = v[0][r].x;
+        b0[r] = v[0][r].y;
+        a1[r] = v[1][r].x;
+        b1[r] = v[1][r].y;
+    }
+}
+
+// Split of the forward transform Z = FFT(fa + i fb) into the two half spectra
+// oa, ob (ob may be null). Only k < kx <= N/2 + 1 is stored, so the mirrors
+// N - k that the split needs are 0 (k = 0, its own mirror) or lie in [N/2, N):
+// registers r = 2, 3. Each lane writes just those two values, at the slots its
+// own last stage read, so the write needs no barrier; Z_k itself comes from
+// the registers. The caller puts a barrier between r2c_mirror and r2c_split.
+template <int N, typename T>
+__device__ __forceinline__ void r2c_mirror(const cpx<T> (&v)[4], cpx<T>* line, int lane) {
     using P = LanePad<T>;
     constexpr int LR = LaneFft<N>::LR;
-    cpx<T> v[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = cpx<T>{fa[r], fb[r]};
-    lane_fft<N, -1, T>(v, line, tw, lane);
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) *(line + P::at(lane + r * LR)) = v[r];
-    __syncthreads();
+    *(line + P::at(lane + 2 * LR)) = v[2];
+    *(line + P::at(lane + 3 * LR)) = v[3];
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void r2c_split(const cpx<T> (&v)[4], const cpx<T>* line, int lane, int kx, cpx<T>* oa,
+                                          cpx<T>* ob) {
+    using P = LanePad<T>;
+    constexpr int LR = LaneFft<N>::LR;
     const T h = T(0.5);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 3; ++r) {
         const int k = lane + r * LR;
         if (k < kx) {
-            const cpx<T> zk = *(line + P::at(k));
-            const cpx<T> zn = *(line + P::at((N - k) & (N - 1)));
+            const cpx<T> zk = v[r];
+            const cpx<T> zn = k == 0 ? zk : *(line + P::at(N - k));
             oa[k] = cpx<T>{h * (zk.x + zn.x), h * (zk.y - zn.y)};
             if (ob) ob[k] = cpx<T>{h * (zk.y + zn.y), h * (zn.x - zk.x)};
         }
     }
+}
+
+// forward transform of one pair (fa, fb) -> (oa, ob)
+template <int N, typename T>
+__device__ __forceinline__ void r2c_pair(const T (&fa)[4], const T (&fb)[4], cpx<T>* oa, cpx<T>* ob, int kx,
+                                         cpx<T>* const (&line)[1], const cpx<T>* tw, int lane) {
+    cpx<T> v[1][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[0][r] = cpx<T>{fa[r], fb[r]};
+    lane_fft<N, -1, 1, T>(v, line, tw, lane);
+    r2c_mirror<N, T>(v[0], line[0], lane);
+    __syncthreads();
+    r2c_split<N, T>(v[0], line[0], lane, kx, oa, ob);
+}
+
+// two forward transforms at once: (fa0, fb0) -> (oa0, ob0), (fa1, fb1) -> (oa1, ob1)
+template <int N, typename T>
+__device__ __forceinline__ void r2c_pair2(const T (&fa0)[4], const T (&fb0)[4], cpx<T>* oa0, cpx<T>* ob0,
+                                          const T (&fa1)[4], const T (&fb1)[4], cpx<T>* oa1, cpx<T>* ob1, int kx,
+                                          cpx<T>* const (&line)[2], const cpx<T>* tw, int lane) {
+    cpx<T> v[2][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        v[0][r] = cpx<T>{fa0[r], fb0[r]};
+        v[1][r] = cpx<T>{fa1[r], fb1[r]};
+    }
+    lane_fft<N, -1, 2, T>(v, line, tw, lane);
+    r2c_mirror<N, T>(v[0], line[0], lane);
+    r2c_mirror<N, T>(v[1], line[1], lane);
+    __syncthreads();
+    r2c_split<N, T>(v[0], line[0], lane, kx, oa0, ob0);
+    r2c_split<N, T>(v[1], line[1], lane, kx, oa1, ob1);
 }
 
 template <typename T, int N, bool FIRST, bool CFL, bool HALL, bool AEB>
@@ -164,7 +239,7 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     constexpr int NT = LaneFft<N>::NT;
     constexpr int RPB = NT / LR;  // rows per block
     constexpr int LS = LanePad<T>::line(N);
-    __shared__ cpx<T> sh[RPB * LS];
+    __shared__ cpx<T> sh[2 * RPB * LS];
     __shared__ cpx<T> stw[N];
     for (int i = threadIdx.x; i < N; i += NT) stw[i] = twg[i];
     __syncthreads();
@@ -174,7 +249,8 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     long long row = (long long)blockIdx.x * RPB + rib;
     const bool valid = row < a.nrows;
     if (!valid) row = a.nrows - 1;  // computes a duplicate row, never stored
-    cpx<T>* line = sh + rib * LS;
+    cpx<T>* const line1[1] = {sh + 2 * rib * LS};
+    cpx<T>* const line2[2] = {sh + 2 * rib * LS, sh + (2 * rib + 1) * LS};
 
     // ---- inputs: rho, m(3), B(3), e [, J(3)] at x = lane + r*LR
     T rho[4], mx[4], my[4], mz[4], bx[4], by[4], bz[4], e[4], jx[4], jy[4], jz[4];
@@ -192,10 +268,8 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     } else {
         const cpx<T>* st = (const cpx<T>*)a.st + row * a.rs_s;
         const long long fs = a.fs_s;
-        c2r_pair<N, T>(st, st + fs, a.kx_s, line, tw, lane, rho, mx);
-        c2r_pair<N, T>(st + 2 * fs, st + 3 * fs, a.kx_s, line, tw, lane, my, mz);
-        c2r_pair<N, T>(st + 4 * fs, st + 5 * fs, a.kx_s, line, tw, lane, bx, by);
-        c2r_pair<N, T>(st + 6 * fs, st + 7 * fs, a.kx_s, line, tw, lane, bz, e);
+        c2r_pair2<N, T>(st, st + fs, st + 2 * fs, st + 3 * fs, a.kx_s, line2, tw, lane, rho, mx, my, mz);
+        c2r_pair2<N, T>(st + 4 * fs, st + 5 * fs, st + 6 * fs, st + 7 * fs, a.kx_s, line2, tw, lane, bx, by, bz, e);
         const T g1 = T(pp.gamma) - T(1);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -229,9 +303,8 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     if constexpr (HALL) {
         const cpx<T>* jb = (const cpx<T>*)a.jb + row * a.rs_j;
         const long long fs = a.fs_j;
-        c2r_pair<N, T>(jb, jb + fs, a.kx_j, line, tw, lane, jx, jy);
         T dummy[4];
-        c2r_pair<N, T>(jb + 2 * fs, nullptr, a.kx_j, line, tw, lane, jz, dummy);
+        c2r_pair2<N, T>(jb, jb + fs, jb + 2 * fs, nullptr, a.kx_j, line2, tw, lane, jx, jy, jz, dummy);
     }
 
     // ---- fluxes (mhdrhs.f90:60-119), kept as point values
@@ -260,38 +333,42 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     cpx<T>* out = (cpx<T>*)a.out + row * a.rs_o;
     const long long fo = a.fs_o;
     const int kxo = valid ? a.kx_o : 0;
-    T fa[4], fb[4];
-    // pairs ordered so that the inputs used by few fluxes die first
-    if constexpr (AEB) {
+    // two transforms at a time; pairs ordered so that the inputs used by few fluxes die first
+    auto O = [&](int f) { return out + f * fo; };
+    T f0[4], f1[4], f2[4], f3[4];
+    // (F13, F14) with (F15, F16)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) fb[r] = T(0);
-        r2c_pair<N, T>(f19, fb, out + 18 * fo, nullptr, kxo, line, tw, lane);
+    for (int r = 0; r < 4; ++r) { f0[r] = ez[r]; f1[r] = ep[r] * ux[r] - udb[r] * bx[r]; }
+    r2c_pair2<N, T>(ex, ey, O(12), O(13), f0, f1, O(14), O(15), kxo, line2, tw, lane);
+    // (F17, F18) with (F4, F5)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        f0[r] = ep[r] * uy[r] - udb[r] * by[r]; f1[r] = ep[r] * uz[r] - udb[r] * bz[r];
+        f2[r] = mx[r] * ux[r] - bx[r] * bx[r] + ptot[r]; f3[r] = my[r] * ux[r] - by[r] * bx[r];
     }
-    // (F13, F14), (F15, F16), (F17, F18)
-    r2c_pair<N, T>(ex, ey, out + 12 * fo, out + 13 * fo, kxo, line, tw, lane);
+    r2c_pair2<N, T>(f0, f1, O(16), O(17), f2, f3, O(3), O(4), kxo, line2, tw, lane);
+    // (F6, F7) with (F8, F9)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) { fa[r] = ez[r]; fb[r] = ep[r] * ux[r] - udb[r] * bx[r]; }
-    r2c_pair<N, T>(fa, fb, out + 14 * fo, out + 15 * fo, kxo, line, tw, lane);
+    for (int r = 0; r < 4; ++r) {
+        f0[r] = mz[r] * ux[r] - bz[r] * bx[r]; f1[r] = mx[r] * uy[r] - bx[r] * by[r];
+        f2[r] = my[r] * uy[r] - by[r] * by[r] + ptot[r]; f3[r] = mz[r] * uy[r] - bz[r] * by[r];
+    }
+    r2c_pair2<N, T>(f0, f1, O(5), O(6), f2, f3, O(7), O(8), kxo, line2, tw, lane);
+    // (F10, F11) with (F12, F1)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) { fa[r] = ep[r] * uy[r] - udb[r] * by[r]; fb[r] = ep[r] * uz[r] - udb[r] * bz[r]; }
-    r2c_pair<N, T>(fa, fb, out + 16 * fo, out + 17 * fo, kxo, line, tw, lane);
-    // (F4, F5), (F6, F7), (F8, F9), (F10, F11), (F12, F1), (F2, F3)
+    for (int r = 0; r < 4; ++r) {
+        f0[r] = mx[r] * uz[r] - bx[r] * bz[r]; f1[r] = my[r] * uz[r] - by[r] * bz[r];
+        f2[r] = mz[r] * uz[r] - bz[r] * bz[r] + ptot[r];
+    }
+    r2c_pair2<N, T>(f0, f1, O(9), O(10), f2, mx, O(11), O(0), kxo, line2, tw, lane);
+    if constexpr (AEB) {
+        // (F2, F3) with the expanding-box energy source F19
 #pragma unroll
-    for (int r = 0; r < 4; ++r) { fa[r] = mx[r] * ux[r] - bx[r] * bx[r] + ptot[r]; fb[r] = my[r] * ux[r] - by[r] * bx[r]; }
-    r2c_pair<N, T>(fa, fb, out + 3 * fo, out + 4 * fo, kxo, line, tw, lane);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) { fa[r] = mz[r] * ux[r] - bz[r] * bx[r]; fb[r] = mx[r] * uy[r] - bx[r] * by[r]; }
-    r2c_pair<N, T>(fa, fb, out + 5 * fo, out + 6 * fo, kxo, line, tw, lane);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) { fa[r] = my[r] * uy[r] - by[r] * by[r] + ptot[r]; fb[r] = mz[r] * uy[r] - bz[r] * by[r]; }
-    r2c_pair<N, T>(fa, fb, out + 7 * fo, out + 8 * fo, kxo, line, tw, lane);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) { fa[r] = mx[r] * uz[r] - bx[r] * bz[r]; fb[r] = my[r] * uz[r] - by[r] * bz[r]; }
-    r2c_pair<N, T>(fa, fb, out + 9 * fo, out + 10 * fo, kxo, line, tw, lane);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) { fa[r] = mz[r] * uz[r] - bz[r] * bz[r] + ptot[r]; fb[r] = mx[r]; }
-    r2c_pair<N, T>(fa, fb, out + 11 * fo, out, kxo, line, tw, lane);
-    r2c_pair<N, T>(my, mz, out + fo, out + 2 * fo, kxo, line, tw, lane);
+        for (int r = 0; r < 4; ++r) f0[r] = T(0);
+        r2c_pair2<N, T>(my, mz, O(1), O(2), f19, f0, O(18), nullptr, kxo, line2, tw, lane);
+    } else {
+        r2c_pair<N, T>(my, mz, O(1), O(2), kxo, line1, tw, lane);
+    }
 }
 
 template <typename T, int N, bool FIRST, bool CFL, bool HALL>
