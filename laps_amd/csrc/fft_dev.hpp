@@ -202,7 +202,7 @@ struct LdsTable {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int t = (int)threadIdx.x + i * NT;
-            if (t < N) s[t] = r[i];
+            if (N % NT == 0 || t < N) s[t] = r[i];
         }
     }
 };

@@ -118,25 +118,41 @@ __device__ __forceinline__ void lane_fft(cpx<T> (&v)[K][4], cpx<T>* const (&line
     }
 }
 
-// Z_p, p = lane + r*LR, of the paired C2R (Z = A + iB with Hermitian
-// extension; only the real parts of the DC and Nyquist coefficients enter, as
-// in FFTW's c2r). Every lane loads A and B at the clamped index min(m, 0 if
-// m >= kx) of the mode m = min(p, N - p) it needs, with no branch around the
-// loads, so the eight loads of a pair are all in flight before the first use.
-template <int N, typename T>
-__device__ __forceinline__ void c2r_inputs(const cpx<T>* __restrict__ A, const cpx<T>* __restrict__ B, int kx,
-                                           int lane, cpx<T> (&v)[4]) {
+// Raw spectral values that lane `lane` needs for two paired C2Rs (A0 + iB0
+// and A1 + iB1): for position p = lane + r*LR, the coefficients at mode
+// m = min(p, N - p), loaded at the clamped index (m < kx ? m : 0) with no
+// branch around the loads. c2r_load only issues the loads, so a kernel can
+// start the next pair's loads before it transforms the current one.
+template <typename T>
+struct C2RRaw {
+    cpx<T> a[2][4], b[2][4];
+};
+
+// HAS_B1 = false: the second transform has no B (its imaginary rows are zero)
+template <int N, bool HAS_B1, typename T>
+__device__ __forceinline__ void c2r_load(const cpx<T>* __restrict__ A0, const cpx<T>* __restrict__ B0,
+                                         const cpx<T>* __restrict__ A1, const cpx<T>* __restrict__ B1, int kx,
+                                         int lane, C2RRaw<T>& q) {
     constexpr int LR = LaneFft<N>::LR;
-    const cpx<T> zero{T(0), T(0)};
-    cpx<T> a[4], b[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int p = lane + r * LR;
         const int m = r < 2 ? p : N - p;  // r < 2: p < N/2; r >= 2: p >= N/2
         const int mc = m < kx ? m : 0;
-        a[r] = A[mc];
-        b[r] = B ? B[mc] : zero;
+        q.a[0][r] = A0[mc];
+        q.b[0][r] = B0[mc];
+        q.a[1][r] = A1[mc];
+        q.b[1][r] = HAS_B1 ? B1[mc] : cpx<T>{T(0), T(0)};
     }
+}
+
+// Z_p of the paired C2R (Z = A + iB with Hermitian extension; only the real
+// parts of the DC and Nyquist coefficients enter, as in FFTW's c2r).
+template <int N, typename T>
+__device__ __forceinline__ void c2r_combine(const cpx<T> (&a)[4], const cpx<T> (&b)[4], int kx, int lane,
+                                            cpx<T> (&v)[4]) {
+    constexpr int LR = LaneFft<N>::LR;
+    const cpx<T> zero{T(0), T(0)};
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int p = lane + r * LR;
@@ -149,15 +165,13 @@ __device__ __forceinline__ void c2r_inputs(const cpx<T>* __restrict__ A, const c
     }
 }
 
-// Two paired C2Rs at once: spectra (A0, B0) to real rows (a0, b0) and
-// (A1, B1) to (a1, b1).
+// The two paired C2Rs of q at once: real rows (a0, b0) and (a1, b1).
 template <int N, typename T>
-__device__ __forceinline__ void c2r_pair2(const cpx<T>* A0, const cpx<T>* B0, const cpx<T>* A1, const cpx<T>* B1,
-                                          int kx, cpx<T>* const (&line)[2], const cpx<T>* tw, int lane, T (&a0)[4],
-                                          T (&b0)[4], T (&a1)[4], T (&b1)[4]) {
+__device__ __forceinline__ void c2r_pair2(const C2RRaw<T>& q, int kx, cpx<T>* const (&line)[2], const cpx<T>* tw,
+                                          int lane, T (&a0)[4], T (&b0)[4], T (&a1)[4], T (&b1)[4]) {
     cpx<T> v[2][4];
-    c2r_inputs<N, T>(A0, B0, kx, lane, v[0]);
-    c2r_inputs<N, T>(A1, B1, kx, lane, v[1]);
+    c2r_combine<N, T>(q.a[0], q.b[0], kx, lane, v[0]);
+    c2r_combine<N, T>(q.a[1], q.b[1], kx, lane, v[1]);
     lane_fft<N, +1, 2, T>(v, line, tw, lane);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -241,8 +255,6 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     constexpr int LS = LanePad<T>::line(N);
     __shared__ cpx<T> sh[2 * RPB * LS];
     __shared__ cpx<T> stw[N];
-    for (int i = threadIdx.x; i < N; i += NT) stw[i] = twg[i];
-    __syncthreads();
     const cpx<T>* tw = stw;
     const int lane = threadIdx.x % LR;
     const int rib = threadIdx.x / LR;
@@ -251,10 +263,23 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     if (!valid) row = a.nrows - 1;  // computes a duplicate row, never stored
     cpx<T>* const line1[1] = {sh + 2 * rib * LS};
     cpx<T>* const line2[2] = {sh + 2 * rib * LS, sh + (2 * rib + 1) * LS};
+    // the twiddle table is fetched first and written to LDS once the row's first
+    // loads are in flight, so the wait for it does not wait for them
+    LdsTable<cpx<T>, N, NT> twt;
+    twt.fetch(twg);
+    auto stage_twiddles = [&]() {
+        twt.put(stw);
+        __syncthreads();
+    };
 
-    // ---- inputs: rho, m(3), B(3), e [, J(3)] at x = lane + r*LR
+    // ---- inputs: rho, m(3), B(3), e [, J(3)] at x = lane + r*LR. Each group
+    // of two paired C2Rs has its loads issued before the previous group's
+    // transforms run, so HBM latency overlaps the FFT work.
     T rho[4], mx[4], my[4], mz[4], bx[4], by[4], bz[4], e[4], jx[4], jy[4], jz[4];
     T ux[4], uy[4], uz[4], p[4];
+    C2RRaw<T> qa, qb;
+    const cpx<T>* jb = (const cpx<T>*)a.jb + row * a.rs_j;
+    const long long fsj = a.fs_j;
     if constexpr (FIRST) {
         const long long N3 = a.npts;
         const double* S = a.S + row * N;
@@ -265,11 +290,17 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
             bx[r] = T(S[4 * N3 + x]); by[r] = T(S[5 * N3 + x]); bz[r] = T(S[6 * N3 + x]); e[r] = T(S[7 * N3 + x]);
             ux[r] = T(S[8 * N3 + x]); uy[r] = T(S[9 * N3 + x]); uz[r] = T(S[10 * N3 + x]); p[r] = T(S[11 * N3 + x]);
         }
+        if constexpr (HALL) c2r_load<N, false, T>(jb, jb + fsj, jb + 2 * fsj, nullptr, a.kx_j, lane, qa);
+        stage_twiddles();
     } else {
         const cpx<T>* st = (const cpx<T>*)a.st + row * a.rs_s;
         const long long fs = a.fs_s;
-        c2r_pair2<N, T>(st, st + fs, st + 2 * fs, st + 3 * fs, a.kx_s, line2, tw, lane, rho, mx, my, mz);
-        c2r_pair2<N, T>(st + 4 * fs, st + 5 * fs, st + 6 * fs, st + 7 * fs, a.kx_s, line2, tw, lane, bx, by, bz, e);
+        c2r_load<N, true, T>(st, st + fs, st + 2 * fs, st + 3 * fs, a.kx_s, lane, qa);
+        stage_twiddles();
+        c2r_load<N, true, T>(st + 4 * fs, st + 5 * fs, st + 6 * fs, st + 7 * fs, a.kx_s, lane, qb);
+        c2r_pair2<N, T>(qa, a.kx_s, line2, tw, lane, rho, mx, my, mz);
+        if constexpr (HALL) c2r_load<N, false, T>(jb, jb + fsj, jb + 2 * fsj, nullptr, a.kx_j, lane, qa);
+        c2r_pair2<N, T>(qb, a.kx_s, line2, tw, lane, bx, by, bz, e);
         const T g1 = T(pp.gamma) - T(1);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -301,10 +332,8 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     }
 
     if constexpr (HALL) {
-        const cpx<T>* jb = (const cpx<T>*)a.jb + row * a.rs_j;
-        const long long fs = a.fs_j;
         T dummy[4];
-        c2r_pair2<N, T>(jb, jb + fs, jb + 2 * fs, nullptr, a.kx_j, line2, tw, lane, jx, jy, jz, dummy);
+        c2r_pair2<N, T>(qa, a.kx_j, line2, tw, lane, jx, jy, jz, dummy);
     }
 
     // ---- fluxes (mhdrhs.f90:60-119), kept as point values
