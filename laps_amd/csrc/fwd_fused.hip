@@ -24,7 +24,8 @@ template <typename T, int N>
 struct FwdCfg {
     using P = PlanLow<N>;
     using I = PlanInfo<N, P>;
-    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
+    // columns per block: 8 for FP32 as well keeps blocks at 8 waves, so several fit on a CU
+    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : 8;
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
@@ -162,8 +163,11 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
     twt.put(stw);  // visible after the barrier that opens each transform
     wzt.put(swz);
     const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
-    const int tile = (int)(blockIdx.x % ntile);
-    const int cy = (int)(blockIdx.x / ntile);
+    // block -> (variable v, kx tile, cy): the eight conserved variables of a
+    // column tile are independent, one block each
+    const int v = (int)(blockIdx.x % 8);
+    const int tile = (int)((blockIdx.x / 8) % ntile);
+    const int cy = (int)(blockIdx.x / 8 / ntile);
     const int col = tile * C::TX + l;
     const bool valid = col < a.KX;
     const int kx = valid ? col : 0;
@@ -177,7 +181,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
     const long long zls = (long long)a.CY * a.rs;   // z line stride of D
     const long long mode0 = (long long)cy * a.rs + kx;
 
-    for (int v = 0; v < 8; ++v) {
+    {
         const ZVar V = a.v[v];
         cpx<T> acc[I::RMAX], w[I::RMAX];
 #pragma unroll
@@ -213,7 +217,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
             for (int r = 0; r < I::RLAST; ++r) *lds(j + r * I::BLAST) = acc[r];
         }
         __syncthreads();
-        if (!valid) continue;
+        if (!valid) return;
 #pragma unroll 1
         for (int p = j; p < N; p += C::TPL) {
             const int cz = pos_to_stored(p, N, a.CZ);
@@ -294,7 +298,7 @@ static hipError_t do_zr(const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const 
                         T scale, hipStream_t s) {
     using C = FwdCfg<T, N>;
     const int ntile = (a.KX + C::TX - 1) / C::TX;
-    const long long nblk = (long long)ntile * a.CY;
+    const long long nblk = 8LL * ntile * a.CY;
     hipLaunchKernelGGL((k_zfwd_rhs<T, N, OPT>), dim3((unsigned)nblk), dim3(C::NT), 0, s, D, U, Fp, a, tw, tb, wzf, pp,
                        ctl, irk, ntile, scale);
     return hipGetLastError();
