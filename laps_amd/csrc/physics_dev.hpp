@@ -5,38 +5,51 @@
 
 namespace laps {
 
-// dt candidate of one point (vardt, mhd.f90:355-409), FP64
-__device__ __forceinline__ double cfl_point(double rho, double Bx, double By, double Bz, double p, double ux,
-                                            double uy, double uz, const PhysParams& pp, double rr, double dmin) {
-    const double cs2 = pp.gamma * p / rho;
-    const double sr = sqrt(rho);
-    const double cax = Bx / sr, cay = By / sr, caz = Bz / sr;
-    const double ca2 = cax * cax + cay * cay + caz * caz;
-    const double cms2 = cs2 + ca2;
-    const double cnx = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cax * cax, 0.0));
-    const double cny = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cay * cay, 0.0));
-    const double cnz = sqrt(fmax(cms2 * cms2 - 4 * cs2 * caz * caz, 0.0));
-    const double s2 = sqrt(2.0);
-    const double cfx = sqrt(cms2 + cnx) / s2, cfy = sqrt(cms2 + cny) / s2, cfz = sqrt(cms2 + cnz) / s2;
-    const double csx = sqrt(fmax(cms2 - cnx, 0.0)) / s2;
-    const double csy = sqrt(fmax(cms2 - cny, 0.0)) / s2;
-    const double csz = sqrt(fmax(cms2 - cnz, 0.0)) / s2;
-    auto cmax = [](double u, double cf, double cs, double ca) {
+// quotient in the arithmetic R of cfl_point: correctly rounded in FP64; the
+// hardware reciprocal-based quotient (about 2 ulp) in FP32
+template <typename R>
+__device__ __forceinline__ R cfl_div(R a, R b) {
+    if constexpr (sizeof(R) == 4) return __fdividef(a, b);
+    else return a / b;
+}
+
+// dt candidate of one point (vardt, mhd.f90:355-409). R = double follows the
+// reference's FP64 arithmetic; the mixed-precision step evaluates it in FP32
+// (R = float), where the fields themselves are FP32.
+template <typename R>
+__device__ __forceinline__ R cfl_point(R rho, R Bx, R By, R Bz, R p, R ux, R uy, R uz, const PhysParams& pp, R rr,
+                                       R dmin) {
+    const R cs2 = cfl_div(R(pp.gamma) * p, rho);
+    const R sr = sqrt(rho);
+    const R cax = cfl_div(Bx, sr), cay = cfl_div(By, sr), caz = cfl_div(Bz, sr);
+    const R ca2 = cax * cax + cay * cay + caz * caz;
+    const R cms2 = cs2 + ca2;
+    const R zero = R(0);
+    const R cnx = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cax * cax, zero));
+    const R cny = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cay * cay, zero));
+    const R cnz = sqrt(fmax(cms2 * cms2 - 4 * cs2 * caz * caz, zero));
+    const R s2 = sqrt(R(2));
+    const R cfx = cfl_div(sqrt(cms2 + cnx), s2), cfy = cfl_div(sqrt(cms2 + cny), s2);
+    const R cfz = cfl_div(sqrt(cms2 + cnz), s2);
+    const R csx = cfl_div(sqrt(fmax(cms2 - cnx, zero)), s2);
+    const R csy = cfl_div(sqrt(fmax(cms2 - cny, zero)), s2);
+    const R csz = cfl_div(sqrt(fmax(cms2 - cnz, zero)), s2);
+    auto cmax = [](R u, R cf, R cs, R ca) {
         return fmax(fmax(fmax(fabs(u + cf), fabs(u + cs)), fmax(fabs(u + ca), fabs(u - cf))),
                     fmax(fmax(fabs(u - cs), fabs(u - ca)), fabs(u)));
     };
-    double cmx = cmax(ux, cfx, csx, cax), cmy = cmax(uy, cfy, csy, cay), cmz = cmax(uz, cfz, csz, caz);
+    R cmx = cmax(ux, cfx, csx, cax), cmy = cmax(uy, cfy, csy, cay), cmz = cmax(uz, cfz, csz, caz);
     if (pp.hall) {
-        double bmax = Bx;  // signed max of the components (mhd.f90:396-398)
+        R bmax = Bx;  // signed max of the components (mhd.f90:396-398)
         if (By > bmax) bmax = By;
         if (Bz > bmax) bmax = Bz;
-        const double ch = pp.di / rho * bmax / dmin;
+        const R ch = cfl_div(cfl_div(R(pp.di), rho) * bmax, dmin);
         cmx = fmax(cmx, ch);
         cmy = fmax(cmy, ch);
         cmz = fmax(cmz, ch);
     }
-    const double dtx = pp.dx / cmx, dty = pp.dy / cmy * rr, dtz = pp.dz / cmz * rr;
-    double d = dtx < dty ? dtx : dty;
+    const R dtx = cfl_div(R(pp.dx), cmx), dty = cfl_div(R(pp.dy), cmy) * rr, dtz = cfl_div(R(pp.dz), cmz) * rr;
+    R d = dtx < dty ? dtx : dty;
     return dtz < d ? dtz : d;
 }
 

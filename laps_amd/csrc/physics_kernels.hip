@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void k_cfl(const double* __restrict__ U, 
     if (pp.dz < dmin) dmin = pp.dz;
     double dtloc = __longlong_as_double(0x7FF0000000000000LL);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
-        const double d = cfl_point(U[i], U[4 * N + i], U[5 * N + i], U[6 * N + i], P[3 * N + i], P[i], P[N + i],
+        const double d = cfl_point<double>(U[i], U[4 * N + i], U[5 * N + i], U[6 * N + i], P[3 * N + i], P[i], P[N + i],
                                    P[2 * N + i], pp, rr, dmin);
         dtloc = fmin(dtloc, d);
     }

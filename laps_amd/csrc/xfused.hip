@@ -312,19 +312,18 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
                                      bz[r] * bz[r])) * g1;
         }
     }
-    // ---- CFL minimum of the first substage (vardt, mhd.f90:352-416)
+    // ---- CFL minimum of the first substage (vardt, mhd.f90:352-416), in the
+    // arithmetic of the pass (FP64, or FP32 in the mixed-precision step)
     if constexpr (CFL) {
-        const double rr = ctl->radius / pp.radius0;
+        const T rr = T(ctl->radius / pp.radius0);
         double dmin = pp.dx < pp.dy ? pp.dx : pp.dy;
         if (pp.dz < dmin) dmin = pp.dz;
         double dtl = __longlong_as_double(0x7FF0000000000000LL);
         if (valid) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const double d = cfl_point(double(rho[r]), double(bx[r]), double(by[r]), double(bz[r]),
-                                               double(p[r]), double(ux[r]), double(uy[r]), double(uz[r]), pp, rr,
-                                               dmin);
-                dtl = fmin(dtl, d);
+                const T d = cfl_point<T>(rho[r], bx[r], by[r], bz[r], p[r], ux[r], uy[r], uz[r], pp, rr, T(dmin));
+                dtl = fmin(dtl, double(d));
             }
         }
         for (int o = 32; o > 0; o >>= 1) dtl = fmin(dtl, __shfl_xor(dtl, o));
