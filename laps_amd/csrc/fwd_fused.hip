@@ -93,6 +93,59 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
     };
 
     cpx<T> va[I::RMAX], vb[I::RMAX];
+    if (G.kind == 3) {
+        // symmetric momentum flux tensor M = (xx, xy, xz, yy, yz, zz) in in[0..5]:
+        // out = -i(kx' Y M_xx + ky' Y M_xy), Y M_xz, -i(kx' Y M_xy + ky' Y M_yy), Y M_yz,
+        //       -i(kx' Y M_xz + ky' Y M_yz), Y M_zz  (the x, y, z momentum divergence groups;
+        // each off-diagonal component is transformed once and used by both rows)
+        auto neg = [](cpx<T> z) { return cpx<T>{-z.x, -z.y}; };
+        const bool h = fft_field(G.in[0], va);                                      // xx
+        T kxr[I::RLAST], kyr[I::RLAST];  // swy is visible after the transform's first barrier
+#pragma unroll
+        for (int r = 0; r < I::RLAST; ++r) {
+            double kxp, kyp;
+            kprime_xy(wx, swy[h ? j + r * I::BLAST : 0], pp, radius, kxp, kyp);
+            kxr[r] = T(kxp);
+            kyr[r] = T(kyp);
+        }
+#pragma unroll
+        for (int r = 0; r < I::RLAST; ++r) va[r] = mul_i(kxr[r], va[r]);
+        fft_field(G.in[1], vb);                                                     // xy
+        if (h) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) {
+                store(G.out[0], j + r * I::BLAST, neg(va[r] + mul_i(kyr[r], vb[r])));
+                vb[r] = mul_i(kxr[r], vb[r]);                                       // y row, x part
+            }
+        }
+        fft_field(G.in[3], va);                                                     // yy
+        if (h) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) store(G.out[2], j + r * I::BLAST, neg(vb[r] + mul_i(kyr[r], va[r])));
+        }
+        fft_field(G.in[2], va);                                                     // xz
+        if (h) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) {
+                store(G.out[1], j + r * I::BLAST, va[r]);
+                va[r] = mul_i(kxr[r], va[r]);                                       // z row, x part
+            }
+        }
+        fft_field(G.in[4], vb);                                                     // yz
+        if (h) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) {
+                store(G.out[3], j + r * I::BLAST, vb[r]);
+                store(G.out[4], j + r * I::BLAST, neg(va[r] + mul_i(kyr[r], vb[r])));
+            }
+        }
+        fft_field(G.in[5], va);                                                     // zz
+        if (h) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) store(G.out[5], j + r * I::BLAST, va[r]);
+        }
+        return;
+    }
     if (G.kind == 2) {
         if (fft_field(G.in[0], va)) {
 #pragma unroll
@@ -145,7 +198,7 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
 template <typename T, int N, int OPT>
 __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* __restrict__ D,
                                                                cpx<double>* __restrict__ U,
-                                                               cpx<double>* __restrict__ Fp, ZArgs a,
+                                                               cpx<T>* __restrict__ Fp, ZArgs a,
                                                                const cpx<T>* __restrict__ tw, BoxTables tb,
                                                                const double* __restrict__ wzf, PhysParams pp,
                                                                const Ctl* __restrict__ ctl, int irk, int ntile,
@@ -240,10 +293,11 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
             if (irk == 0) {
                 un = cpx<double>{c * fnl.x + 0.0, c * fnl.y + 0.0} + u;
             } else {
-                const cpx<double> fp = Fp[v * a.ufs + mi];
+                const cpx<T> fpt = Fp[v * a.ufs + mi];
+                const cpx<double> fp{double(fpt.x), double(fpt.y)};
                 un = cpx<double>{c * fnl.x + d * fp.x, c * fnl.y + d * fp.y} + u;
             }
-            Fp[v * a.ufs + mi] = fnl;
+            Fp[v * a.ufs + mi] = cpx<T>{T(fnl.x), T(fnl.y)};
             if (pp.visc && !pp.visc_exp && v >= 1 && v <= 3) {
                 const double den = ts * k2 * pp.nu + 1.0;
                 un = {un.x / den, un.y / den};
@@ -293,7 +347,7 @@ hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs
 }
 
 template <typename T, int N, int OPT>
-static hipError_t do_zr(const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a, const cpx<T>* tw,
+static hipError_t do_zr(const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, const ZArgs& a, const cpx<T>* tw,
                         const BoxTables& tb, const double* wzf, const PhysParams& pp, const Ctl* ctl, int irk,
                         T scale, hipStream_t s) {
     using C = FwdCfg<T, N>;
@@ -305,7 +359,7 @@ static hipError_t do_zr(const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const 
 }
 
 template <typename T, int N>
-static hipError_t do_zr_opt(const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a, const cpx<T>* tw,
+static hipError_t do_zr_opt(const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, const ZArgs& a, const cpx<T>* tw,
                             const BoxTables& tb, const double* wzf, const PhysParams& pp, const Ctl* ctl, int irk,
                             T scale, hipStream_t s) {
     if (pp.dealias == 1) return do_zr<T, N, 1>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
@@ -314,7 +368,7 @@ static hipError_t do_zr_opt(const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, co
 }
 
 template <typename T>
-hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a,
+hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, const ZArgs& a,
                            const cpx<T>* tw, const BoxTables& tb, const double* wzf, const PhysParams& pp,
                            const Ctl* ctl, int irk, T scale, hipStream_t s) {
     switch (nz) {
@@ -337,7 +391,7 @@ template hipError_t launch_yfwd_group<float>(int, const cpx<float>*, cpx<float>*
 template hipError_t launch_zfwd_rhs<double>(int, const cpx<double>*, cpx<double>*, cpx<double>*, const ZArgs&,
                                             const cpx<double>*, const BoxTables&, const double*, const PhysParams&,
                                             const Ctl*, int, double, hipStream_t);
-template hipError_t launch_zfwd_rhs<float>(int, const cpx<float>*, cpx<double>*, cpx<double>*, const ZArgs&,
+template hipError_t launch_zfwd_rhs<float>(int, const cpx<float>*, cpx<double>*, cpx<float>*, const ZArgs&,
                                            const cpx<float>*, const BoxTables&, const double*, const PhysParams&,
                                            const Ctl*, int, float, hipStream_t);
 

@@ -106,9 +106,10 @@ hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cp
 
 // --- forward y / z passes with the RHS and RK3 folded in (fwd_fused.hip) ---
 struct YGroup {
-    int kind;     // 0 divergence (3 in, 2 out), 1 induction (3 in, 5 out), 2 copy (1 in, 1 out)
-    int in[3];
-    int out[5];
+    int kind;     // 0 divergence (3 in, 2 out), 1 induction (3 in, 5 out), 2 copy (1 in, 1 out),
+                  // 3 symmetric momentum flux tensor (6 in, 6 out)
+    int in[6];
+    int out[6];
 };
 struct YGArgs {
     int ncol, cy;
@@ -138,7 +139,7 @@ hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs
                              const double* wxt, const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz,
                              hipStream_t s);
 template <typename T>
-hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<double>* Fp, const ZArgs& a,
+hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, const ZArgs& a,
                            const cpx<T>* tw, const BoxTables& tb, const double* wzf, const PhysParams& pp,
                            const Ctl* ctl, int irk, T scale, hipStream_t s);
 

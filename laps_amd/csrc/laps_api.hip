@@ -6,7 +6,9 @@
 //                                         Hall current J_hat (3), kept box
 //   Ufull  [11][nz][ny][nx/2+1] FP64     : un-dealiased spectrum of the state
 //                                         handed over by laps_set_state
-//   Fp     [8][CZ][CY][KX] FP64 complex  : previous RHS fnl_rk
+//   Fp     [8][CZ][CY][KX] complex       : previous RHS fnl_rk, stored in the
+//                                         transform type (FP64, or FP32 in
+//                                         mixed precision)
 //   S      [12][nz][ny][nx] FP64 real    : real-space state uu(8)+uu_prim(4)
 //                                         (the uploaded one, or materialised)
 //   P1,P2  T complex                     : transform intermediates, T = FP64
@@ -164,7 +166,7 @@ struct laps_ctx {
     }
     long long kb() const { return kept.modes(); }
     int nf_in() const { return p.if_hall ? 11 : 8; }
-    int nf_out() const { return p.if_aeb ? 19 : 18; }
+    int nf_out() const { return p.if_aeb ? 16 : 15; }  // x pass fluxes: 3 + 6 (symmetric tensor) + 3 + 3 (+1)
 };
 
 namespace {
@@ -452,9 +454,10 @@ int substage(laps_ctx* c, int irk) {
     }));
     const ZArgs& za = aeb ? c->zr_aeb : c->zr;
     const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
-    const double zb = c->zfwd_bytes * sizeof(T) + 16.0 * KB * (24 + (irk ? 8 : 0));
+    // U read + write (FP64), Fp write (+ read when irk > 0) in the transform type
+    const double zb = c->zfwd_bytes * sizeof(T) + 8.0 * KB * (32.0 + 2.0 * sizeof(T) * (irk ? 2 : 1));
     LAPS_CK(timed(c, K_ZFWD, zb, [&] {
-        return launch_zfwd_rhs<T>(c->nz, P2, c->U, c->Fp, za, tw_of<T>(c, 2), c->tkept.view(), c->tfull.wz, c->pp,
+        return launch_zfwd_rhs<T>(c->nz, P2, c->U, (cpx<T>*)c->Fp, za, tw_of<T>(c, 2), c->tkept.view(), c->tfull.wz, c->pp,
                                   c->ctl, irk, scale, c->stream);
     }));
     return LAPS_OK;
@@ -575,22 +578,31 @@ void setup_forward_groups(laps_ctx* c) {
         y.rs = k.KXP;
         y.in_fs = NZ * NY * k.KXP;
         y.out_fs = NZ * k.CY * k.KXP;
-        const int div_in[5][3] = {{0, 1, 2}, {3, 4, 5}, {6, 7, 8}, {9, 10, 11}, {15, 16, 17}};
-        const int div_out[5][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {13, 14}};
+        // x pass flux slots: 0-2 mass, 3-8 momentum tensor (xx xy xz yy yz zz), 9-11 EMF,
+        // 12-14 energy, 15 AEB source. Output slots (z pass terms): mass 0,1; momentum
+        // x 2,3, y 4,5, z 6,7; induction 8-12; energy 13,14; AEB 15.
         int g = 0;
-        for (int d = 0; d < 5; ++d, ++g) {
+        const int div_in[2][3] = {{0, 1, 2}, {12, 13, 14}};
+        const int div_out[2][2] = {{0, 1}, {13, 14}};
+        for (int d = 0; d < 2; ++d, ++g) {
             y.g[g].kind = 0;
             for (int t = 0; t < 3; ++t) y.g[g].in[t] = div_in[d][t];
             y.g[g].out[0] = div_out[d][0];
             y.g[g].out[1] = div_out[d][1];
         }
+        y.g[g].kind = 3;
+        for (int t = 0; t < 6; ++t) {
+            y.g[g].in[t] = 3 + t;
+            y.g[g].out[t] = 2 + t;
+        }
+        ++g;
         y.g[g].kind = 1;
-        y.g[g].in[0] = 12; y.g[g].in[1] = 13; y.g[g].in[2] = 14;
+        y.g[g].in[0] = 9; y.g[g].in[1] = 10; y.g[g].in[2] = 11;
         y.g[g].out[0] = 11; y.g[g].out[1] = 9; y.g[g].out[2] = 8; y.g[g].out[3] = 10; y.g[g].out[4] = 12;
         ++g;
         if (withaeb) {
             y.g[g].kind = 2;
-            y.g[g].in[0] = 18;
+            y.g[g].in[0] = 15;
             y.g[g].out[0] = 15;
             ++g;
         }

@@ -1,5 +1,5 @@
 // xfused.hip -- the x pass of a substage in one kernel: x C2R of the state
-// (and Hall current), primitives, the 18(+1) nonlinear fluxes, the CFL
+// (and Hall current), primitives, the 15(+1) nonlinear fluxes, the CFL
 // minimum, and x R2C of the fluxes. The real-space fields never reach HBM.
 //
 // Reference: calc_flux (mhdrhs.f90:21-124), update_uu_prim_from_uu
@@ -361,41 +361,40 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     cpx<T>* out = (cpx<T>*)a.out + row * a.rs_o;
     const long long fo = a.fs_o;
     const int kxo = valid ? a.kx_o : 0;
-    // two transforms at a time; pairs ordered so that the inputs used by few fluxes die first
+    // Output slots: 0-2 mass flux (rho u), 3-8 the symmetric momentum flux tensor
+    // M_ij = (rho u_i) u_j - B_i B_j + ptot d_ij as xx, xy, xz, yy, yz, zz, 9-11 the EMF,
+    // 12-14 the energy flux, 15 the expanding-box energy source. The reference's 18
+    // fluxes (mhdrhs.f90:60-119) hold both m_j u_i and m_i u_j off the diagonal; they are
+    // equal up to rounding, so each is formed once, with the reference's expression of
+    // its first-row use (xy: my ux - By Bx, xz: mz ux - Bz Bx, yz: mz uy - Bz By).
+    // Two transforms at a time; pairs ordered so that the inputs used by few fluxes die first.
     auto O = [&](int f) { return out + f * fo; };
     T f0[4], f1[4], f2[4], f3[4];
-    // (F13, F14) with (F15, F16)
+    // (E_x, E_y) with (E_z, energy flux x)
 #pragma unroll
     for (int r = 0; r < 4; ++r) { f0[r] = ez[r]; f1[r] = ep[r] * ux[r] - udb[r] * bx[r]; }
-    r2c_pair2<N, T>(ex, ey, O(12), O(13), f0, f1, O(14), O(15), kxo, line2, tw, lane);
-    // (F17, F18) with (F4, F5)
+    r2c_pair2<N, T>(ex, ey, O(9), O(10), f0, f1, O(11), O(12), kxo, line2, tw, lane);
+    // (energy flux y, z) with (M_xx, M_xy)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         f0[r] = ep[r] * uy[r] - udb[r] * by[r]; f1[r] = ep[r] * uz[r] - udb[r] * bz[r];
         f2[r] = mx[r] * ux[r] - bx[r] * bx[r] + ptot[r]; f3[r] = my[r] * ux[r] - by[r] * bx[r];
     }
-    r2c_pair2<N, T>(f0, f1, O(16), O(17), f2, f3, O(3), O(4), kxo, line2, tw, lane);
-    // (F6, F7) with (F8, F9)
+    r2c_pair2<N, T>(f0, f1, O(13), O(14), f2, f3, O(3), O(4), kxo, line2, tw, lane);
+    // (M_xz, M_yy) with (M_yz, M_zz)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        f0[r] = mz[r] * ux[r] - bz[r] * bx[r]; f1[r] = mx[r] * uy[r] - bx[r] * by[r];
-        f2[r] = my[r] * uy[r] - by[r] * by[r] + ptot[r]; f3[r] = mz[r] * uy[r] - bz[r] * by[r];
+        f0[r] = mz[r] * ux[r] - bz[r] * bx[r]; f1[r] = my[r] * uy[r] - by[r] * by[r] + ptot[r];
+        f2[r] = mz[r] * uy[r] - bz[r] * by[r]; f3[r] = mz[r] * uz[r] - bz[r] * bz[r] + ptot[r];
     }
     r2c_pair2<N, T>(f0, f1, O(5), O(6), f2, f3, O(7), O(8), kxo, line2, tw, lane);
-    // (F10, F11) with (F12, F1)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        f0[r] = mx[r] * uz[r] - bx[r] * bz[r]; f1[r] = my[r] * uz[r] - by[r] * bz[r];
-        f2[r] = mz[r] * uz[r] - bz[r] * bz[r] + ptot[r];
-    }
-    r2c_pair2<N, T>(f0, f1, O(9), O(10), f2, mx, O(11), O(0), kxo, line2, tw, lane);
+    // (mass flux x, y) with (mass flux z, expanding-box source or nothing)
     if constexpr (AEB) {
-        // (F2, F3) with the expanding-box energy source F19
+        r2c_pair2<N, T>(mx, my, O(0), O(1), mz, f19, O(2), O(15), kxo, line2, tw, lane);
+    } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) f0[r] = T(0);
-        r2c_pair2<N, T>(my, mz, O(1), O(2), f19, f0, O(18), nullptr, kxo, line2, tw, lane);
-    } else {
-        r2c_pair<N, T>(my, mz, O(1), O(2), kxo, line1, tw, lane);
+        r2c_pair2<N, T>(mx, my, O(0), O(1), mz, f0, O(2), nullptr, kxo, line2, tw, lane);
     }
 }
 
