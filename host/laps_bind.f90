@@ -96,6 +96,12 @@ module laps_c
             real(c_double), intent(out) :: ave(8), var(8), rho_u2(3)
         end function
 
+        integer(c_int) function laps_budgets(ctx, out) bind(C, name='laps_budgets')
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out(7)
+        end function
+
         integer(c_int) function laps_max_divb(ctx, out) bind(C, name='laps_max_divb')
             import :: c_ptr, c_int, c_double
             type(c_ptr), value :: ctx

@@ -413,7 +413,7 @@ contains
     ! calc_max_divB + output_rms + output_AEB (mhd.f90:183-192; mhdrms.f90:39-51; AEBmod.f90:75-85)
     subroutine output_rms_all(t)
         real(dp), intent(in) :: t
-        real(dp) :: ave(8), var(8), ru2(3), tt, ddt, rr
+        real(dp) :: ave(8), var(8), ru2(3), tt, ddt, rr, bud(7)
         integer(c_int64_t) :: ist
         integer(c_int32_t) :: st
         integer :: u
@@ -438,6 +438,17 @@ contains
             open(newunit=u, file='EBM_info.dat', status='replace', action='write')
         endif
         write(u, '(3(1pe16.8))') t, rr, Ur0
+        close(u)
+        ! budgets (no reference counterpart): <e>, kinetic, magnetic, internal energy,
+        ! cross, magnetic and current helicity, one line per rms output
+        call laps_check(laps_budgets(ctx, bud), 'laps_budgets')
+        inquire(file='budgets.dat', exist=ex)
+        if (ex) then
+            open(newunit=u, file='budgets.dat', status='old', action='write', position='append')
+        else
+            open(newunit=u, file='budgets.dat', status='replace', action='write')
+        endif
+        write(u, '(f12.6,2x,7(1pe20.12))') t, bud
         close(u)
     end subroutine output_rms_all
 

@@ -136,6 +136,16 @@ int laps_get_spectral(laps_ctx* ctx, double* uhat);
  * variances <f^2>-<f>^2 and <rho (u_i - <u_i>)^2>, reduced in FP64. */
 int laps_moments(laps_ctx* ctx, double ave[8], double var[8], double rho_u2[3]);
 
+/* Budgets; no counterpart in the reference, whose diagnostics are calc_rms
+ * and calc_max_divB (mhdrms.f90:53-126, mhd.f90:522-570). Volume averages in
+ * FP64: out[0] <e>, out[1] kinetic <(rho u).u>/2, out[2] magnetic <|B|^2>/2,
+ * out[3] internal <p>/(gamma-1), out[4] cross helicity <u.B>, and by Parseval
+ * over the spectrum without k = 0: out[5] magnetic helicity <A.B> with the
+ * Coulomb-gauge potential A_hat = i k x B_hat / |k|^2, out[6] current
+ * helicity <J.B>, J_hat = i k x B_hat (k with the expanding-box scaling). */
+#define LAPS_NBUDGET 7
+int laps_budgets(laps_ctx* ctx, double out[LAPS_NBUDGET]);
+
 /* calc_max_divB (mhd.f90:522-570): max over modes of |k . B_hat|. */
 int laps_max_divb(laps_ctx* ctx, double* out);
 

@@ -157,6 +157,10 @@ hipError_t launch_moments(const double* U8, const double* P4, long long npts, do
                           hipStream_t s);
 hipError_t launch_moments2(const double* U8, const double* P4, long long npts, const double* ave, double* partial,
                            int nblk, hipStream_t s);
+hipError_t launch_budget_real(const double* U8, const double* P4, long long npts, double gamma, double* partial,
+                              int nblk, hipStream_t s);
+hipError_t launch_budget_spec(const cpx<double>* U, long long fs, Box b, BoxTables t, const PhysParams& pp,
+                              const Ctl* ctl, int nx, double* partial, int nblk, hipStream_t s);
 hipError_t launch_divb(const cpx<double>* U, long long fstride, Box b, BoxTables t, const PhysParams& pp,
                        const Ctl* ctl, unsigned long long* out, hipStream_t s);
 hipError_t launch_gather_box(const cpx<double>* full, cpx<double>* box, int nf, long long full_stride,

@@ -989,6 +989,38 @@ int laps_moments(laps_ctx* c, double ave[8], double var[8], double rho_u2[3]) {
     return LAPS_OK;
 }
 
+int laps_budgets(laps_ctx* c, double out[LAPS_NBUDGET]) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!out) return fail(LAPS_EINVAL, "null output");
+    if ((rc = materialize(c))) return rc;
+    const int nb = c->nblk_red;
+    LAPS_CK(launch_budget_real(c->S, c->S + 8 * c->N, c->N, c->p.adiabatic_index, c->red, nb, c->stream));
+    std::vector<double> part((size_t)nb * 5);
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(part.data(), c->red, part.size() * 8, hipMemcpyDeviceToHost));
+    double r[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 5; ++k) r[k] += part[(size_t)b * 5 + k];
+    for (int k = 0; k < 5; ++k) out[k] = r[k] / (double)c->N;
+    if ((rc = refresh_fresh(c))) return rc;
+    if (c->fresh)
+        LAPS_CK(launch_budget_spec(c->Ufull, c->M, c->full, c->tfull.view(), c->pp, c->ctl, c->nx, c->red, nb,
+                                   c->stream));
+    else
+        LAPS_CK(launch_budget_spec(c->U, c->kb(), c->kept, c->tkept.view(), c->pp, c->ctl, c->nx, c->red, nb,
+                                   c->stream));
+    std::vector<double> part2((size_t)nb * 2);
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(part2.data(), c->red, part2.size() * 8, hipMemcpyDeviceToHost));
+    double h[2] = {0, 0};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 2; ++k) h[k] += part2[(size_t)b * 2 + k];
+    out[5] = h[0];
+    out[6] = h[1];
+    return LAPS_OK;
+}
+
 int laps_max_divb(laps_ctx* c, double* out) {
     int rc = check_ctx(c, true);
     if (rc) return rc;

@@ -281,6 +281,82 @@ hipError_t launch_divb(const cpx<double>* U, long long fs, Box b, BoxTables t, c
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ budgets
+// Per-block sums over the real state: e, (m . u)/2, |B|^2/2, p/(gamma-1), u . B.
+__global__ __launch_bounds__(kThreads) void k_budget_real(const double* __restrict__ U, const double* __restrict__ P,
+                                                          long long N, double gm1, double* __restrict__ part) {
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        const double mx = U[N + i], my = U[2 * N + i], mz = U[3 * N + i];
+        const double bx = U[4 * N + i], by = U[5 * N + i], bz = U[6 * N + i];
+        const double ux = P[i], uy = P[N + i], uz = P[2 * N + i];
+        s[0] += U[7 * N + i];
+        s[1] += 0.5 * (mx * ux + my * uy + mz * uz);
+        s[2] += 0.5 * (bx * bx + by * by + bz * bz);
+        s[3] += P[3 * N + i] / gm1;
+        s[4] += ux * bx + uy * by + uz * bz;
+    }
+    __shared__ double red[5][kThreads];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Per-block sums over the stored half spectrum of w_k Re(conj(A_k) . B_k) and
+// w_k Re(conj(J_k) . B_k), with J = i k' x B and the Coulomb-gauge potential
+// A = i k' x B / |k'|^2 (k' with the AEB / corotating scaling, as in curl);
+// k = 0 is left out (A is undefined there) and w_k = 2 for 0 < kx < nx/2, else 1.
+__global__ __launch_bounds__(kThreads) void k_budget_spec(const cpx<double>* __restrict__ U, long long fs, Box b,
+                                                          BoxTables t, PhysParams pp, const Ctl* __restrict__ ctl,
+                                                          int nx, double* __restrict__ part) {
+    const long long n = b.modes();
+    const double radius = ctl->radius;
+    double hm = 0.0, hj = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int kx, cy, cz;
+        if (!box_index(i, b, kx, cy, cz)) continue;
+        const KVec k = kvec(t.wx[kx], t.wy[cy], t.wz[cz], pp, radius);
+        const double k2 = k.kx * k.kx + k.ky * k.ky + k.kz * k.kz;
+        if (k2 == 0.0) continue;
+        const cpx<double> bx = U[4 * fs + i], by = U[5 * fs + i], bz = U[6 * fs + i];
+        // J = i k x B
+        const cpx<double> jx = ik(k.ky, bz) - ik(k.kz, by);
+        const cpx<double> jy = ik(k.kz, bx) - ik(k.kx, bz);
+        const cpx<double> jz = ik(k.kx, by) - ik(k.ky, bx);
+        const double jb = jx.x * bx.x + jx.y * bx.y + jy.x * by.x + jy.y * by.y + jz.x * bz.x + jz.y * bz.y;
+        const double w = (kx == 0 || 2 * kx == nx) ? 1.0 : 2.0;
+        hj += w * jb;
+        hm += w * jb / k2;
+    }
+    __shared__ double red[2][kThreads];
+    red[0][threadIdx.x] = hm;
+    red[1][threadIdx.x] = hj;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 2; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) part[blockIdx.x * 2 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t launch_budget_real(const double* U8, const double* P4, long long npts, double gamma, double* partial,
+                              int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_budget_real, dim3(nblk), dim3(kThreads), 0, s, U8, P4, npts, gamma - 1.0, partial);
+    return hipGetLastError();
+}
+hipError_t launch_budget_spec(const cpx<double>* U, long long fs, Box b, BoxTables t, const PhysParams& pp,
+                              const Ctl* ctl, int nx, double* partial, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_budget_spec, dim3(nblk), dim3(kThreads), 0, s, U, fs, b, t, pp, ctl, nx, partial);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ full spectrum -> kept box
 __global__ void k_gather_box(const cpx<double>* __restrict__ full, cpx<double>* __restrict__ box, int nf,
                              long long fstride, long long bstride, int nxh, int ny, int nz, Box b) {

@@ -21,7 +21,7 @@ _D = ctypes.POINTER(ctypes.c_double)
 EXPORTS = (
     "laps_default_params", "laps_create", "laps_destroy", "laps_last_error", "laps_set_state",
     "laps_set_time", "laps_set_fixed_dt", "laps_cfl", "laps_step", "laps_get_scalars", "laps_get_state",
-    "laps_get_spectral", "laps_moments", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
+    "laps_get_spectral", "laps_moments", "laps_budgets", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
     "laps_timing_enable", "laps_timing_read", "laps_get_stream",
 )
 
@@ -52,6 +52,7 @@ def lib():
         L.laps_get_spectral.argtypes = [vp, _D]
         L.laps_moments.argtypes = [vp, _D, _D, _D]
         L.laps_max_divb.argtypes = [vp, _D]
+        L.laps_budgets.argtypes = [vp, _D]
         L.laps_fft_r2c.argtypes = [vp, ctypes.c_int, _D, _D]
         L.laps_fft_c2r.argtypes = [vp, ctypes.c_int, _D, _D]
         L.laps_sync.argtypes = [vp]
@@ -148,6 +149,15 @@ class Laps:
         ave, var, r = np.zeros(8), np.zeros(8), np.zeros(3)
         _check(lib().laps_moments(self._h, _fptr(ave), _fptr(var), _fptr(r)))
         return ave, var, r
+
+    BUDGETS = ("e", "kinetic", "magnetic", "internal", "cross_helicity", "magnetic_helicity",
+               "current_helicity")
+
+    def budgets(self):
+        """Volume averages: <e>, <rho u.u>/2, <B.B>/2, <p>/(gamma-1), <u.B>, <A.B>, <J.B>."""
+        out = np.zeros(len(self.BUDGETS))
+        _check(lib().laps_budgets(self._h, _fptr(out)))
+        return dict(zip(self.BUDGETS, out.tolist()))
 
     def max_divb(self):
         d = ctypes.c_double()
