@@ -166,6 +166,10 @@ struct oracle {
     double *filtx, *filty, *filtz;
     double *uu, *prim, *flux, *J;
     cx *uhat, *fhat, *jhat, *fnl, *fnl_rk;
+    /* phase-shift dealiasing (option 3): e^{i pi m/n} per axis and the shifted-grid buffers */
+    cx *phx, *phy, *phz;
+    double *uu_s, *prim_s;
+    cx *uhat_s, *fhat_s;
     double time, dt, dt_last, radius, Ur, tau, cos_c, sin_c, fixed_dt;
     double cc1[3], dd1[3], tstep[3];
     int64_t istep;
@@ -275,6 +279,30 @@ oracle* oracle_create(const laps_params* p) {
     for (int i = 0; i < o->ny; ++i) o->filty[i] = compact_filter(o->wy[i], p->Ly, p->ny, p->afy);
     for (int i = 0; i < o->nz; ++i) o->filtz[i] = compact_filter(o->wz[i], p->Lz, p->nz, p->afz);
     o->fixed_dt = 0.0;
+    if (p->dealias_option == 3) {
+        /* half a cell along every axis: e^{i k delta} = e^{i pi m/n} per axis, signed m
+         * with the +Nyquist convention of the wave numbers */
+        o->phx = (cx*)malloc(sizeof(cx) * o->nxh);
+        o->phy = (cx*)malloc(sizeof(cx) * o->ny);
+        o->phz = (cx*)malloc(sizeof(cx) * o->nz);
+        o->uu_s = (double*)calloc(8 * o->N, sizeof(double));
+        o->prim_s = (double*)calloc(4 * o->N, sizeof(double));
+        o->uhat_s = (cx*)calloc(8 * o->M, sizeof(cx));
+        o->fhat_s = (cx*)calloc(19 * o->M, sizeof(cx));
+        if (!o->phx || !o->phy || !o->phz || !o->uu_s || !o->prim_s || !o->uhat_s || !o->fhat_s) {
+            oracle_destroy(o);
+            return NULL;
+        }
+        for (int i = 0; i < o->nxh; ++i) { o->phx[i].re = cos(o->pi * i / o->nx); o->phx[i].im = sin(o->pi * i / o->nx); }
+        for (int i = 0; i < o->ny; ++i) {
+            int m = (i <= o->ny / 2) ? i : i - o->ny;
+            o->phy[i].re = cos(o->pi * m / o->ny); o->phy[i].im = sin(o->pi * m / o->ny);
+        }
+        for (int i = 0; i < o->nz; ++i) {
+            int m = (i <= o->nz / 2) ? i : i - o->nz;
+            o->phz[i].re = cos(o->pi * m / o->nz); o->phz[i].im = sin(o->pi * m / o->nz);
+        }
+    }
     return o;
 }
 
@@ -284,6 +312,7 @@ void oracle_destroy(oracle* o) {
     free(o->filtx); free(o->filty); free(o->filtz);
     free(o->uu); free(o->prim); free(o->flux); free(o->J);
     free(o->uhat); free(o->fhat); free(o->jhat); free(o->fnl); free(o->fnl_rk);
+    free(o->phx); free(o->phy); free(o->phz); free(o->uu_s); free(o->prim_s); free(o->uhat_s); free(o->fhat_s);
     free(o);
 }
 
@@ -483,6 +512,51 @@ static void transform_flux(oracle* o) {
     oracle_fft_r2c(o, nf, o->flux, (double*)o->fhat);
 }
 
+static void update_prim(oracle* o);
+
+/* out = in * (e^{i pi mx/nx} e^{i pi my/ny} e^{i pi mz/nz})^sign, nf fields */
+static void phase_shift(const oracle* o, const cx* in, cx* out, int nf, int sign) {
+    const size_t M = o->M;
+#pragma omp parallel for collapse(2) num_threads(nthreads())
+    for (int iz = 0; iz < o->nz; ++iz)
+        for (int iy = 0; iy < o->ny; ++iy)
+            for (int ix = 0; ix < o->nxh; ++ix) {
+                cx ph = cx_mul(cx_mul(o->phx[ix], o->phy[iy]), o->phz[iz]);
+                if (sign < 0) ph.im = -ph.im;
+                size_t s = SDX3(o, ix, iy, iz);
+                for (int v = 0; v < nf; ++v) out[v * M + s] = cx_mul(in[v * M + s], ph);
+            }
+}
+
+/* Phase-shift dealiasing (Patterson & Orszag 1971; no reference counterpart):
+ * the flux spectra are the average of the unshifted evaluation (already in
+ * fhat) and the evaluation on the grid shifted by half a cell along every
+ * axis, brought back with the conjugate phase. With the sqrt(2)/3 ellipsoid
+ * this removes the aliasing of every quadratic product. */
+static void shifted_flux(oracle* o) {
+    const size_t M = o->M;
+    const int nf = o->p.if_aeb ? 19 : 18;
+    cx* scratch = (cx*)malloc(sizeof(cx) * M);
+    phase_shift(o, o->uhat, o->uhat_s, 8, +1);
+    for (int v = 0; v < 8; ++v) inv3d(o, o->uhat_s + (size_t)v * M, o->uu_s + (size_t)v * o->N, scratch);
+    free(scratch);
+    double* t;
+    cx* c;
+    t = o->uu; o->uu = o->uu_s; o->uu_s = t;
+    t = o->prim; o->prim = o->prim_s; o->prim_s = t;
+    c = o->uhat; o->uhat = o->uhat_s; o->uhat_s = c;
+    c = o->fhat; o->fhat = o->fhat_s; o->fhat_s = c;   /* fhat_s: the unshifted spectra */
+    update_prim(o);
+    calc_flux(o);
+    transform_flux(o);
+    t = o->uu; o->uu = o->uu_s; o->uu_s = t;
+    t = o->prim; o->prim = o->prim_s; o->prim_s = t;
+    c = o->uhat; o->uhat = o->uhat_s; o->uhat_s = c;
+    phase_shift(o, o->fhat, o->fhat, nf, -1);
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+    for (size_t i = 0; i < (size_t)nf * M; ++i) o->fhat[i] = cx_scale(cx_add(o->fhat_s[i], o->fhat[i]), 0.5);
+}
+
 /* calc_rhs (mhdrhs.f90:174-279) */
 static void calc_rhs(oracle* o) {
     const size_t M = o->M;
@@ -590,6 +664,23 @@ static void dealias_array(const oracle* o, cx* u) {
                         for (int v = 0; v < 8; ++v) { u[v * M + s].re = 0.0; u[v * M + s].im = 0.0; }
                     }
                 }
+    } else if (o->p.dealias_option == 3) {
+        /* phase-shift dealiasing keeps the open ellipsoid (mx/nx)^2 + (my/ny)^2 + (mz/nz)^2 < 2/9
+         * (no reference counterpart), tested exactly in integers: modes on the boundary can
+         * alias in two directions at once when an n is a multiple of 3 */
+        const long long X = o->nx, Y = o->ny, Z = o->nz;
+#pragma omp parallel for collapse(2) num_threads(nthreads())
+        for (int iz = 0; iz < o->nz; ++iz)
+            for (int iy = 0; iy < o->ny; ++iy)
+                for (int ix = 0; ix < o->nxh; ++ix) {
+                    const long long mx = ix, my = iy <= o->ny / 2 ? iy : iy - o->ny;
+                    const long long mz = iz <= o->nz / 2 ? iz : iz - o->nz;
+                    if (9 * (mx * mx * Y * Y * Z * Z + my * my * X * X * Z * Z + mz * mz * X * X * Y * Y) >=
+                        2 * X * X * Y * Y * Z * Z) {
+                        size_t s = SDX3(o, ix, iy, iz);
+                        for (int v = 0; v < 8; ++v) { u[v * M + s].re = 0.0; u[v * M + s].im = 0.0; }
+                    }
+                }
     } else if (o->p.dealias_option == 2) {
         for (int v = 0; v < 8; ++v)
 #pragma omp parallel for collapse(2) num_threads(nthreads())
@@ -687,6 +778,7 @@ static void evolve(oracle* o) {
     for (int irk = 0; irk < 3; ++irk) {
         calc_flux(o);
         transform_flux(o);
+        if (o->p.dealias_option == 3) shifted_flux(o);
         calc_rhs(o);
         rkt(o, irk);
         dealias_array(o, o->uhat);
@@ -857,6 +949,7 @@ int oracle_max_divb(oracle* o, double* out) {
 int oracle_rhs(oracle* o, double* fnl_out) {
     calc_flux(o);
     transform_flux(o);
+    if (o->p.dealias_option == 3) shifted_flux(o);
     calc_rhs(o);
     memcpy(fnl_out, o->fnl, sizeof(cx) * 8 * o->M);
     return 0;
