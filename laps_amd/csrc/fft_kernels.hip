@@ -48,10 +48,15 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
     cpx<T>* line = sh + l * C::LS;
     const int in_cnt = a.in_cnt, out_cnt = a.out_cnt;
     const long long in_ls = a.in_ls, out_ls = a.out_ls;
+    const cpx<double>* ph = a.ph;
     auto load = [&](int p) -> cpx<T> {
         const int s = pos_to_stored(p, N, in_cnt);
         if (!valid || s < 0) return cpx<T>{T(0), T(0)};
         const cpx<TI> v = ib[s * in_ls];
+        if (ph) {
+            const cpx<double> w = ph[p];
+            return cmul(cpx<T>{T(v.x), T(v.y)}, cpx<T>{T(w.x), T(w.y)});
+        }
         return cpx<T>{T(v.x), T(v.y)};
     };
     auto lds = [&](int p) { return line + p; };

@@ -87,8 +87,10 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
         __syncthreads();  // the LDS line may still be read by the previous transform
         return line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, v);
     };
+    const cpx<double>* ph = a.ph;
     auto store = [&](int f, int p, cpx<T> val) {
         const int s = pos_to_stored(p, N, a.cy);
+        if (ph) val = cmul(val, cpx<T>{T(ph[p].x), T(-ph[p].y)});  // back from the shifted grid
         if (valid && s >= 0) ob[f * a.out_fs + s * a.rs] = val;
     };
 
@@ -237,30 +239,53 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
     {
         const ZVar V = a.v[v];
         cpx<T> acc[I::RMAX], w[I::RMAX];
+        // sum_t m_t Z(base[f_t]) into acc (m_t in {1, -i kz', +i kz'})
+        auto terms = [&](const cpx<T>* base) {
+            bool h = false;
 #pragma unroll
-        for (int r = 0; r < I::RMAX; ++r) acc[r] = cpx<T>{T(0), T(0)};
-        bool has = false;
+            for (int r = 0; r < I::RMAX; ++r) acc[r] = cpx<T>{T(0), T(0)};
 #pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            if (t >= V.nterm) break;
-            const cpx<T>* src = ib + V.t[t].field * a.in_fs;
-            auto load = [&](int p) -> cpx<T> { return valid ? src[p * zls] : cpx<T>{T(0), T(0)}; };
-            __syncthreads();
-            has = line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, w);
-            if (has) {
-                const int m = V.t[t].mult;
+            for (int t = 0; t < 3; ++t) {
+                if (t >= V.nterm) break;
+                const cpx<T>* src = base + V.t[t].field * a.in_fs;
+                auto load = [&](int p) -> cpx<T> { return valid ? src[p * zls] : cpx<T>{T(0), T(0)}; };
+                __syncthreads();
+                h = line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, w);
+                if (h) {
+                    const int m = V.t[t].mult;
 #pragma unroll
-                for (int r = 0; r < I::RLAST; ++r) {
-                    if (m == 0) {
-                        acc[r] = acc[r] + w[r];
-                    } else {
-                        const int p = j + r * I::BLAST;
-                        const T kz = T((swz[p] * pp.radius0) / radius);
-                        const cpx<T> q = mul_i(kz, w[r]);
-                        acc[r] = m > 0 ? acc[r] + q : acc[r] - q;
+                    for (int r = 0; r < I::RLAST; ++r) {
+                        if (m == 0) {
+                            acc[r] = acc[r] + w[r];
+                        } else {
+                            const int p = j + r * I::BLAST;
+                            const T kz = T((swz[p] * pp.radius0) / radius);
+                            const cpx<T> q = mul_i(kz, w[r]);
+                            acc[r] = m > 0 ? acc[r] + q : acc[r] - q;
+                        }
                     }
                 }
             }
+            return h;
+        };
+        bool has;
+        if (a.Ds) {
+            // phase-shift dealiasing: fnl = (Z(D) + conj(ph) Z(Ds)) / 2, ph of the output mode
+            const cpx<T>* ibs = (const cpx<T>*)a.Ds + (long long)cy * a.rs + kx;
+            has = terms(ibs);
+            cpx<T> accs[I::RLAST];
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) {
+                const cpx<double> q = a.ph[has ? j + r * I::BLAST : 0];
+                accs[r] = cmul(acc[r], cpx<T>{T(q.x), T(-q.y)});
+            }
+            has = terms(ib);
+            const T half = T(0.5);
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r)
+                acc[r] = cpx<T>{half * (acc[r].x + accs[r].x), half * (acc[r].y + accs[r].y)};
+        } else {
+            has = terms(ib);
         }
         // stage fnl of this column in the LDS line, then update the kept modes in
         // a rolled loop: adjacent work-items take adjacent kx (coalesced U, Fp)
@@ -362,7 +387,7 @@ template <typename T, int N>
 static hipError_t do_zr_opt(const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, const ZArgs& a, const cpx<T>* tw,
                             const BoxTables& tb, const double* wzf, const PhysParams& pp, const Ctl* ctl, int irk,
                             T scale, hipStream_t s) {
-    if (pp.dealias == 1) return do_zr<T, N, 1>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+    if (pp.dealias == 1 || pp.dealias == 3) return do_zr<T, N, 1>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
     if (pp.dealias == 2) return do_zr<T, N, 2>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
     return do_zr<T, N, 0>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
 }

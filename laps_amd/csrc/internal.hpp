@@ -21,6 +21,7 @@ struct StridedArgs {
     long long in_b1, out_b1;
     int nb2;
     long long in_b2, out_b2;
+    const cpx<double>* ph;    // optional: input position p is multiplied by ph[p] (grid shift)
 };
 
 // Rows of an x transform: `nrows` rows (even), complex rows hold `kx` stored
@@ -99,6 +100,8 @@ struct XFArgs {
     void* out;                // flux spectral rows (T complex)
     int kx_o, rs_o;
     long long fs_o;
+    const cpx<double>* ph;    // optional grid shift: C2R input mode m times ph[m], R2C output mode k
+                              // times conj(ph[k])
 };
 template <typename T>
 hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp,
@@ -118,6 +121,7 @@ struct YGArgs {
     long long out_fs;
     int ngroups;
     YGroup g[8];
+    const cpx<double>* ph;    // optional grid shift: output mode p times conj(ph[p])
 };
 struct ZTerm {
     int field;
@@ -133,6 +137,8 @@ struct ZArgs {
     long long in_fs;
     long long ufs;
     ZVar v[8];
+    const void* Ds;           // optional second input (T complex, layout of D) from the shifted grid:
+    const cpx<double>* ph;    // fnl = (Z(D) + conj(ph[p]) Z(Ds)) / 2 at output mode p
 };
 template <typename T>
 hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,

@@ -134,6 +134,8 @@ struct laps_ctx {
     double* S = nullptr;
     void* P1 = nullptr;
     void* P2 = nullptr;
+    void* P3 = nullptr;                                   // shifted-grid y-pass output (option 3)
+    cpx<double>*phx = nullptr, *phy = nullptr, *phz = nullptr;  // e^{i pi m/n} per axis (option 3)
     cpx<double>* X1 = nullptr;
     cpx<double>* X2 = nullptr;
     double* Xr = nullptr;
@@ -211,10 +213,14 @@ int upload_tables(laps_ctx* c, const Box& b, bool pruned, const std::vector<int>
 }
 
 // Bounding box of the modes kept by the 2/3 rule (dealiasing.f90:152-164),
-// evaluated with the reference's expression, plus the per-column |mz| limit.
+// evaluated with the reference's expression, or by phase-shift dealiasing
+// (option 3, no reference counterpart: the open ellipsoid
+// (mx/nx)^2 + (my/ny)^2 + (mz/nz)^2 < 2/9, tested exactly in integers), plus
+// the per-column |mz| limit.
 int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
     const int nx = c->nx, ny = c->ny, nz = c->nz, nxh = c->nxh;
-    if (c->p.dealias_option != 1) {
+    const bool shift = c->p.dealias_option == 3;
+    if (c->p.dealias_option != 1 && !shift) {
         c->kept = c->full;
         c->kept.KXP = (c->full.KX + 15) / 16 * 16;
         return LAPS_OK;
@@ -224,6 +230,11 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
     for (int i = 0; i < ny; ++i) { const double w = wave_number(i, ny, c->p.Ly); ay[i] = w * c->p.Ly / (2 * kPiRef * ny); }
     for (int i = 0; i < nz; ++i) { const double w = wave_number(i, nz, c->p.Lz); az[i] = w * c->p.Lz / (2 * kPiRef * nz); }
     auto keep = [&](int ix, int iy, int iz) {
+        if (shift) {
+            const long long X = nx, Y = ny, Z = nz, mx = ix, my = signed_index(iy, ny), mz = signed_index(iz, nz);
+            return 9 * (mx * mx * Y * Y * Z * Z + my * my * X * X * Z * Z + mz * mz * X * X * Y * Y) <
+                   2 * X * X * Y * Y * Z * Z;
+        }
         const double r = sqrt(ax[ix] * ax[ix] + ay[iy] * ay[iy] + az[iz] * az[iz]);
         return !(r > 1. / 3.);
     };
@@ -237,7 +248,7 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
             // the kept set of a column must be |mz| <= lim exactly
             for (int iz = 0; iz < nz; ++iz) {
                 const bool inside = std::abs(signed_index(iz, nz)) <= lim;
-                if (inside != keep(ix, iy, iz)) return fail(LAPS_EINVAL, "2/3 mask is not a z-interval");
+                if (inside != keep(ix, iy, iz)) return fail(LAPS_EINVAL, "dealias mask is not a z-interval");
             }
             col[(size_t)iy * nxh + ix] = lim;
             if (lim >= 0) {
@@ -246,7 +257,7 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
                 kzmax = std::max(kzmax, lim);
             }
         }
-    if (kxmax < 0) return fail(LAPS_EINVAL, "2/3 mask keeps no mode");
+    if (kxmax < 0) return fail(LAPS_EINVAL, "dealias mask keeps no mode");
     Box b;
     b.KX = kxmax + 1;
     b.KXP = (b.KX + 15) / 16 * 16;
@@ -323,11 +334,15 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 // spectral box (fields with stride fs) -> real fields, unnormalised
 // (transform_uu_fourier_to_real, fftw.f90:73-103)
 // With dst == nullptr the x leg is left out and B holds the y-inverted rows.
+// With shift, every field is sampled on the grid shifted by half a cell along
+// each axis (phase-shift dealiasing): the z and y legs multiply their inputs
+// by e^{i pi m/n}; the x leg is the caller's.
 template <typename T>
 int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
-                 cpx<T>* B, int plan_z = 0, int plan_y = 0) {
+                 cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false) {
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
+    z.ph = shift ? c->phz : nullptr;
     const long long RS = b.KXP;
     z.ncol = b.KX; z.in_cnt = b.CZ; z.out_cnt = (int)NZ;
     z.in_ls = (long long)b.CY * RS; z.out_ls = (long long)b.CY * RS;
@@ -337,6 +352,7 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
     LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + NZ * 2.0 * sizeof(T)),
                   [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
+    y.ph = shift ? c->phy : nullptr;
     y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
     y.in_ls = RS; y.out_ls = RS;
     y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * RS; y.out_b1 = NY * RS;
@@ -452,10 +468,49 @@ int substage(laps_ctx* c, int irk) {
         return launch_yfwd_group<T>(c->ny, P1, P2, ya, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
                                     c->nz, c->stream);
     }));
-    const ZArgs& za = aeb ? c->zr_aeb : c->zr;
+    ZArgs za = aeb ? c->zr_aeb : c->zr;
+    if (c->p.dealias_option == 3) {
+        // phase-shift dealiasing: the same chain on the grid shifted by half a cell along every
+        // axis (inputs times e^{i pi m/n} per axis, outputs times the conjugate); its y-pass
+        // output goes to P3 and the z pass averages the two right-hand sides
+        cpx<T>* P3 = (cpx<T>*)c->P3;
+        XFArgs xs = xa;
+        if (first) {
+            if ((rc = inv_pipeline<T>(c, c->Ufull, c->M, c->full, c->nf_in(), (T*)nullptr, P1, P3, 0, 0, true)))
+                return rc;
+            xs.kx_s = c->nxh;
+            xs.rs_s = c->full.KXP;
+            xs.fs_s = NZ * NY * c->full.KXP;
+        } else {
+            if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P3, c->plan_zinv, c->plan_yinv,
+                                      true)))
+                return rc;
+            xs.kx_s = k.KX;
+            xs.rs_s = k.KXP;
+            xs.fs_s = NZ * NY * k.KXP;
+        }
+        xs.st = P3;
+        xs.jb = P3 + 8 * xs.fs_s;
+        xs.kx_j = xs.kx_s;
+        xs.rs_j = xs.rs_s;
+        xs.fs_j = xs.fs_s;
+        xs.ph = c->phx;
+        const double xins = (double)c->nf_in() * xs.nrows * xs.kx_s * 2 * sizeof(T);
+        LAPS_CK(timed(c, K_XFUSED, xins + xout, [&] {
+            return launch_xfused<T>(c->nx, false, false, xs, tw_of<T>(c, 0), c->pp, c->ctl, c->stream);
+        }));
+        YGArgs ys = ya;
+        ys.ph = c->phy;
+        LAPS_CK(timed(c, K_YFWD, c->yfwd_bytes * sizeof(T), [&] {
+            return launch_yfwd_group<T>(c->ny, P1, P3, ys, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
+                                        c->nz, c->stream);
+        }));
+        za.Ds = P3;
+        za.ph = c->phz;
+    }
     const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
     // U read + write (FP64), Fp write (+ read when irk > 0) in the transform type
-    const double zb = c->zfwd_bytes * sizeof(T) + 8.0 * KB * (32.0 + 2.0 * sizeof(T) * (irk ? 2 : 1));
+    const double zb = c->zfwd_bytes * sizeof(T) * (za.Ds ? 2 : 1) + 8.0 * KB * (32.0 + 2.0 * sizeof(T) * (irk ? 2 : 1));
     LAPS_CK(timed(c, K_ZFWD, zb, [&] {
         return launch_zfwd_rhs<T>(c->nz, P2, c->U, (cpx<T>*)c->Fp, za, tw_of<T>(c, 2), c->tkept.view(), c->tfull.wz, c->pp,
                                   c->ctl, irk, scale, c->stream);
@@ -738,7 +793,8 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     if (!supported_len(p->nx) || !supported_len(p->ny) || !supported_len(p->nz))
         return fail(LAPS_EINVAL, "grid lengths must be 16, 32, 64, 128, 256 or 512 per axis");
     if (p->precision != LAPS_FP64 && p->precision != LAPS_MIXED) return fail(LAPS_EINVAL, "unknown precision");
-    if (p->dealias_option < 0 || p->dealias_option > 2) return fail(LAPS_EINVAL, "dealias_option must be 0, 1 or 2");
+    if (p->dealias_option < 0 || p->dealias_option > 3)
+        return fail(LAPS_EINVAL, "dealias_option must be 0, 1, 2 or 3");
     if (!(p->Lx > 0 && p->Ly > 0 && p->Lz > 0)) return fail(LAPS_EINVAL, "box lengths must be positive");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0)
@@ -763,7 +819,8 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     std::vector<int> kzlim;
     if ((rc = compute_kept_box(c, kzlim))) return bail(rc);
     if ((rc = upload_tables(c, c->full, false, {}, c->tfull))) return bail(rc);
-    if ((rc = upload_tables(c, c->kept, p->dealias_option == 1, kzlim, c->tkept))) return bail(rc);
+    if ((rc = upload_tables(c, c->kept, p->dealias_option == 1 || p->dealias_option == 3, kzlim, c->tkept)))
+        return bail(rc);
     // twiddle tables W_n^t for the three axis lengths, FP64 and FP32
     const int lens[3] = {c->nx, c->ny, c->nz};
     for (int a = 0; a < 3; ++a) {
@@ -789,6 +846,11 @@ int laps_create(const laps_params* p, laps_ctx** out) {
                              (long long)nfo * k.CZ * k.CY * k.KXP, 3LL * NZ * NY * c->nxh});
     long long p2 = std::max({(long long)nfi * NZ * NY * k.KXP, (long long)nfo * NZ * k.CY * k.KXP,
                              3LL * NZ * NY * c->nxh});
+    const bool shift = p->dealias_option == 3;
+    if (shift) {  // the shifted first substage inverts the full spectrum (11 fields) through P1 and P3
+        p1 = std::max(p1, (long long)nfi * NZ * NY * c->full.KXP);
+        p2 = std::max(p2, (long long)nfi * NZ * NY * c->full.KXP);
+    }
     const size_t ts = c->mixed ? sizeof(float) : sizeof(double);
     // FP64 single-field scratch: a full spectrum, or one kept-box field inverted along z and y
     const size_t xs = (size_t)std::max(c->M, NZ * NY * (long long)k.KXP);
@@ -800,6 +862,26 @@ int laps_create(const laps_params* p, laps_ctx** out) {
         (rc = c->allocn(&c->red, (size_t)c->nblk_red * 16)) || (rc = c->allocn(&c->dave, 8)) ||
         (rc = c->allocn(&c->dbits, 1)))
         return bail(rc);
+    if (shift) {
+        // half a cell along every axis: e^{i k delta} = e^{i pi m/n}, signed m with the +Nyquist
+        // convention of the wave numbers
+        auto table = [&](cpx<double>** dst, int n, int cnt) -> int {
+            std::vector<cpx<double>> h(cnt);
+            for (int i = 0; i < cnt; ++i) {
+                const int m = signed_index(i, n);
+                h[i].x = cos(kPiRef * m / n);
+                h[i].y = sin(kPiRef * m / n);
+            }
+            int r = c->allocn(dst, cnt);
+            if (r) return r;
+            if (hipMemcpy(*dst, h.data(), cnt * sizeof(cpx<double>), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(LAPS_EDEVICE, "phase table upload failed");
+            return LAPS_OK;
+        };
+        if ((rc = c->alloc(&c->P3, p2 * 2 * ts)) || (rc = table(&c->phx, c->nx, c->nxh)) ||
+            (rc = table(&c->phy, c->ny, c->ny)) || (rc = table(&c->phz, c->nz, c->nz)))
+            return bail(rc);
+    }
     if ((rc = reset_ctl(c, 0.0, 0.0, false))) return bail(rc);
     *out = c;
     return LAPS_OK;

@@ -102,6 +102,8 @@ def test_set_state_spectrum_and_diagnostics(laps_mod, oracle_mod):
     dict(if_aeb=1, radius0=1.0, Ur0=0.5, if_hall=1, ion_inertial_length=0.02),
     dict(if_aeb=1, if_corotating=1, corotating_angle=0.3, radius0=1.0, Ur0=0.5),
     dict(dealias_option=0, if_visc=0, if_resis=0),
+    dict(dealias_option=3, if_hall=1, ion_inertial_length=0.05),   # phase-shift dealiasing
+    dict(dealias_option=3, if_aeb=1, radius0=1.0, Ur0=0.5),
 ])
 def test_steps_fp64_vs_oracle(laps_mod, oracle_mod, case):
     p = params(32, **case)
@@ -125,6 +127,14 @@ def test_noncubic_fp64(laps_mod, oracle_mod):
     assert rel_l2(g.spectral(), o.spectral()) < FP64_TOL
 
 
+def test_phase_shift_noncubic_fp64(laps_mod, oracle_mod):
+    p = params((32, 64, 16), dealias_option=3, if_hall=1, ion_inertial_length=0.05, Ly=3.0)
+    g, o = run_pair(laps_mod, oracle_mod, p, 3, seed=6, kmax=5)
+    err = rel_l2(g.spectral(), o.spectral())
+    print(f"phase shift non-cubic rel_l2={err:.3e}")
+    assert err < FP64_TOL
+
+
 def test_config0_64cube_100_steps(laps_mod, oracle_mod):
     """Plain FP64 MHD, 64^3, shells 1-8 k^-5/3, rms 0.1, nu=eta=1e-3, 2/3 rule, CFL 0.4, 100 steps."""
     p = params(64, viscosity=1e-3, resistivity=1e-3, cfl=0.4)
@@ -139,7 +149,8 @@ def test_config0_64cube_100_steps(laps_mod, oracle_mod):
     check_moments(g.moments(), o.moments(), 1e-10)
 
 
-@pytest.mark.parametrize("case", [dict(), dict(if_hall=1, ion_inertial_length=0.05), dict(dealias_option=2)])
+@pytest.mark.parametrize("case", [dict(), dict(if_hall=1, ion_inertial_length=0.05), dict(dealias_option=2),
+                                  dict(dealias_option=3, if_hall=1, ion_inertial_length=0.05)])
 def test_mixed_precision_vs_fp64_oracle(laps_mod, oracle_mod, case):
     p = params(32, precision=LAPS_MIXED, **case)
     g = laps_mod.Laps(p)
