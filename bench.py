@@ -4,17 +4,21 @@
 Workload (the 256^3 Hall-MHD case): periodic box [0,2pi)^3, gamma = 5/3,
 rho0 = 1, P0 = 1, B0 = (1,0,0), seeded random-phase modes in shells 1-16
 with amplitude |k|^(-5/3), rms |db| = 0.3 B0 and rms |du| = 0.3, d_i = 0.05,
-nu = eta = 2e-4 (implicit), 2/3-rule dealiasing (the reference's option 1),
-CFL 0.4. A step is one full RK3 step (three substages, CFL dt on the device).
+nu = eta = 2e-4 (implicit), phase-shift dealiasing (dealias_option 3: the
+nonlinear terms averaged over the grid and the grid shifted by half a cell,
+open sqrt(2)/3 ellipsoid), CFL 0.4. A step is one full RK3 step (three
+substages, CFL dt on the device).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--dealias 3|1]
 
 Prints one JSON line on stdout: `value` is FP64 steps/s with the state
 resident in HBM; the mixed-precision run (FP32 transforms and nonlinear
 products, FP64 RK3 accumulation) is reported under "mixed" with its relative
-L2 distance to the FP64 run after the same number of steps. `roofline` is
-the dominant kernel's algorithmic bytes over its HIP-event time; `cpu_baseline`
-is the CPU oracle timed on this host for one step of the same workload.
+L2 distance to the FP64 run after the same number of steps. The same two runs
+with the reference's 2/3 rule (dealias_option 1) are reported under
+"dealias_2_3". `roofline` is the dominant kernel's algorithmic bytes over its
+HIP-event time; `cpu_baseline` is the CPU oracle timed on this host for three
+steps of the headline workload.
 """
 import argparse
 import json
@@ -35,10 +39,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def workload_params(n, precision):
+def workload_params(n, precision, dealias=1):
     from laps_amd.params import default_params
     return default_params(nx=n, ny=n, nz=n, Lx=2 * math.pi, Ly=2 * math.pi, Lz=2 * math.pi,
-                          adiabatic_index=5.0 / 3.0, cfl=0.4, dealias_option=1,
+                          adiabatic_index=5.0 / 3.0, cfl=0.4, dealias_option=dealias,
                           if_visc=1, viscosity=2e-4, if_resis=1, resistivity=2e-4,
                           if_hall=1, ion_inertial_length=0.05, precision=precision)
 
@@ -61,10 +65,10 @@ def shell_spectrum(uhat, n):
     return np.bincount(shell, weights=e.ravel())
 
 
-def run_gpu(n, precision, steps, warmup, uu, pr):
+def run_gpu(n, precision, steps, warmup, uu, pr, dealias=1):
     import torch
     from laps_amd.runtime import Laps
-    p = workload_params(n, precision)
+    p = workload_params(n, precision, dealias)
     g = Laps(p)
     g.set_state(uu, pr)
     log(f"precision={'fp64' if precision == 0 else 'mixed'}: warmup {warmup} steps")
@@ -105,10 +109,10 @@ def roofline(timing, precision):
                 launches=nl)
 
 
-def cpu_baseline(n, uu, pr, threads):
+def cpu_baseline(n, uu, pr, threads, dealias):
     from oracle import binding as ob
     ob.set_threads(threads)
-    p = workload_params(n, 0)
+    p = workload_params(n, 0, dealias)
     o = ob.Oracle(p)
     log(f"cpu baseline: oracle set_state at {n}^3 on {threads} threads")
     o.set_state(uu, pr)
@@ -130,6 +134,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--dealias", type=int, choices=[1, 3], default=3,
+                    help="3: phase-shift dealiasing (the workload's own); 1: the reference's 2/3 rule")
+    ap.add_argument("--no-compare", action="store_true", help="skip the runs with the other dealiasing option")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -153,14 +160,43 @@ def main():
     from laps_amd.params import LAPS_FP64, LAPS_MIXED
     log(f"building {n}^3 seeded initial condition")
     uu, pr = initial_state(n, 5.0 / 3.0)
-    res = {}
-    if args.precision in ("fp64", "both"):
-        res["fp64"] = run_gpu(n, LAPS_FP64, args.steps, args.warmup, uu, pr)
-    if args.precision in ("mixed", "both"):
-        res["mixed"] = run_gpu(n, LAPS_MIXED, args.steps, args.warmup, uu, pr)
+    dl = args.dealias
+
+    def runs(dealias):
+        res = {}
+        if args.precision in ("fp64", "both"):
+            res["fp64"] = run_gpu(n, LAPS_FP64, args.steps, args.warmup, uu, pr, dealias)
+        if args.precision in ("mixed", "both"):
+            res["mixed"] = run_gpu(n, LAPS_MIXED, args.steps, args.warmup, uu, pr, dealias)
+        return res
+
+    def mixed_report(res, sps, dealias):
+        m = res["mixed"]
+        a, b = m["spectral"], res["fp64"]["spectral"]
+        rel = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+        ea, eb = shell_spectrum(a, n), shell_spectrum(b, n)
+        kmax = int(n * (math.sqrt(2.0) / 3.0 if dealias == 3 else 1.0 / 3.0))
+        sel = np.arange(len(eb)) <= kmax
+        sel &= eb > 1e-10 * eb[1:].max()      # shells holding energy above round-off level
+        sel[0] = False
+        dev = float(np.max(np.abs(ea[sel] - eb[sel]) / eb[sel]))
+        msps = args.steps / m["seconds"]
+        return dict(value=round(msps, 3), unit="steps/s", speedup_vs_fp64=round(msps / sps, 3),
+                    rel_l2_vs_fp64=rel, steps_compared=args.steps + args.warmup, max_shell_energy_dev=dev,
+                    roofline=roofline(m["timing"], "mixed" if dealias == 1 else "mixed_ps"),
+                    ms_per_step=round(1000.0 * m["seconds"] / args.steps, 4))
+
+    def kernels(r):
+        return {k: dict(ms_per_step=round(v[0] / args.steps, 4), gbs=round(v[1] / (v[0] * 1e-3) / 1e9, 1)
+                        if v[0] > 0 else 0.0, launches=v[2]) for k, v in r["timing"].items() if v[2]}
+
+    dname = {3: "phase-shift dealiasing (half-cell shifted grid, open sqrt(2)/3 ellipsoid)",
+             1: "2/3-rule dealiasing (the reference's option 1)"}
+    res = runs(dl)
     main_key = "fp64" if "fp64" in res else "mixed"
     r = res[main_key]
     sps = args.steps / r["seconds"]
+    tkey = main_key if dl == 1 else main_key + "_ps"
     out = {
         "metric": "RK3 steps/s at 256^3 Hall-MHD, FP64 vs mixed; % of HBM roofline",
         "value": round(sps, 3),
@@ -175,31 +211,30 @@ def main():
         "dtype": "f64" if main_key == "fp64" else "f32/f64",
         "data": "synthetic (seeded k^-5/3 shells 1-16 initial condition)",
         "config": {"workload": f"compressible Hall-MHD RK3 step, {n}^3 periodic, d_i=0.05, nu=eta=2e-4, "
-                               f"2/3-rule dealiasing, CFL 0.4",
-                   "grid": [n, n, n], "precision": main_key, "dealias_option": 1},
-        "roofline": roofline(r["timing"], main_key),
-        "kernels": {k: dict(ms_per_step=round(v[0] / args.steps, 4), gbs=round(v[1] / (v[0] * 1e-3) / 1e9, 1)
-                            if v[0] > 0 else 0.0, launches=v[2]) for k, v in r["timing"].items() if v[2]},
+                               f"{dname[dl]}, CFL 0.4",
+                   "grid": [n, n, n], "precision": main_key, "dealias_option": dl},
+        "roofline": roofline(r["timing"], tkey),
+        "kernels": kernels(r),
     }
     if "fp64" in res and "mixed" in res:
-        m = res["mixed"]
-        a, b = m["spectral"], res["fp64"]["spectral"]
-        rel = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
-        ea, eb = shell_spectrum(a, n), shell_spectrum(b, n)
-        kmax = n // 3
-        sel = np.arange(len(eb)) <= kmax
-        sel &= eb > 1e-10 * eb[1:].max()      # shells holding energy above round-off level
-        sel[0] = False
-        dev = float(np.max(np.abs(ea[sel] - eb[sel]) / eb[sel]))
-        msps = args.steps / m["seconds"]
-        out["mixed"] = dict(value=round(msps, 3), unit="steps/s", speedup_vs_fp64=round(msps / sps, 3),
-                            rel_l2_vs_fp64=rel, steps_compared=args.steps + args.warmup,
-                            max_shell_energy_dev=dev, roofline=roofline(m["timing"], "mixed"),
-                            ms_per_step=round(1000.0 * m["seconds"] / args.steps, 4))
+        out["mixed"] = mixed_report(res, sps, dl)
+    if not args.no_compare and "fp64" in res:
+        other = 1 if dl == 3 else 3
+        ro = runs(other)
+        so = args.steps / ro["fp64"]["seconds"]
+        rep = dict(dealias_option=other, value=round(so, 3), unit="steps/s",
+                   ms_per_step=round(1000.0 * ro["fp64"]["seconds"] / args.steps, 4),
+                   roofline=roofline(ro["fp64"]["timing"], "fp64" if other == 1 else "fp64_ps"),
+                   kernels=kernels(ro["fp64"]))
+        if "mixed" in ro:
+            rep["mixed"] = mixed_report(ro, so, other)
+        for v in ro.values():
+            v.pop("spectral", None)
+        out["dealias_2_3" if other == 1 else "dealias_phase_shift"] = rep
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
-            out["cpu_baseline"] = cpu_baseline(n, uu, pr, threads)
+            out["cpu_baseline"] = cpu_baseline(n, uu, pr, threads, dl)
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline failed: {e}")
             out["cpu_baseline"] = None
