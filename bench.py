@@ -6,8 +6,15 @@ rho0 = 1, P0 = 1, B0 = (1,0,0), seeded random-phase modes in shells 1-16
 with amplitude |k|^(-5/3), rms |db| = 0.3 B0 and rms |du| = 0.3, d_i = 0.05,
 nu = eta = 2e-4 (implicit), phase-shift dealiasing (dealias_option 3: the
 nonlinear terms averaged over the grid and the grid shifted by half a cell,
-open sqrt(2)/3 ellipsoid), CFL 0.4. A step is one full RK3 step (three
+open sqrt(2)/3 ellipsoid), CFL 0.2. A step is one full RK3 step (three
 substages, CFL dt on the device).
+
+CFL 0.2: the Hall limit of the CFL dt is cfl dx^2 rho / (d_i |B|), and the
+whistler at wave number k turns at omega = d_i k^2 |B| / rho, so omega dt =
+cfl (k dx)^2. RK3 is stable for |omega dt| <= sqrt(3). The open sqrt(2)/3
+ellipsoid keeps k_x up to 120 at 256^3 (k dx = 2.95), which needs cfl <= 0.2;
+at 0.4 the highest kept whistlers grow and the dt sequence turns erratic. The
+2/3 rule keeps k_x <= 85 (0.4 would sit at the limit); both options run at 0.2.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--dealias 3|1]
 
@@ -40,10 +47,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def workload_params(n, precision, dealias=1):
+def workload_params(n, precision, dealias=1, cfl=0.2):
     from laps_amd.params import default_params
     return default_params(nx=n, ny=n, nz=n, Lx=2 * math.pi, Ly=2 * math.pi, Lz=2 * math.pi,
-                          adiabatic_index=5.0 / 3.0, cfl=0.4, dealias_option=dealias,
+                          adiabatic_index=5.0 / 3.0, cfl=cfl, dealias_option=dealias,
                           if_visc=1, viscosity=2e-4, if_resis=1, resistivity=2e-4,
                           if_hall=1, ion_inertial_length=0.05, precision=precision)
 
@@ -249,7 +256,7 @@ def main():
         "dtype": "f64" if main_key == "fp64" else "f32/f64",
         "data": "synthetic (seeded k^-5/3 shells 1-16 initial condition)",
         "config": {"workload": f"compressible Hall-MHD RK3 step, {n}^3 periodic, d_i=0.05, nu=eta=2e-4, "
-                               f"{dname[dl]}, CFL 0.4",
+                               f"{dname[dl]}, CFL 0.2",
                    "grid": [n, n, n], "precision": main_key, "dealias_option": dl},
         "roofline": roofline(r["timing"], tkey),
         "kernels": kernels(r),

@@ -49,12 +49,14 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
     const int in_cnt = a.in_cnt, out_cnt = a.out_cnt;
     const long long in_ls = a.in_ls, out_ls = a.out_ls;
     const cpx<double>* ph = a.ph;
+    // grid shift: the column and slab factors are fixed per work-item
+    const cpx<double> wcb = ph ? cmul(a.phc[valid ? col : 0], a.phb[b1]) : cpx<double>{1.0, 0.0};
     auto load = [&](int p) -> cpx<T> {
         const int s = pos_to_stored(p, N, in_cnt);
         if (!valid || s < 0) return cpx<T>{T(0), T(0)};
         const cpx<TI> v = ib[s * in_ls];
         if (ph) {
-            const cpx<double> w = ph[p];
+            const cpx<double> w = cmul(ph[p], wcb);
             return cmul(cpx<T>{T(v.x), T(v.y)}, cpx<T>{T(w.x), T(w.y)});
         }
         return cpx<T>{T(v.x), T(v.y)};

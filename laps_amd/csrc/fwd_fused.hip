@@ -87,10 +87,8 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
         __syncthreads();  // the LDS line may still be read by the previous transform
         return line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, v);
     };
-    const cpx<double>* ph = a.ph;
     auto store = [&](int f, int p, cpx<T> val) {
         const int s = pos_to_stored(p, N, a.cy);
-        if (ph) val = cmul(val, cpx<T>{T(ph[p].x), T(-ph[p].y)});  // back from the shifted grid
         if (valid && s >= 0) ob[f * a.out_fs + s * a.rs] = val;
     };
 
@@ -272,11 +270,12 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
         if (a.Ds) {
             // phase-shift dealiasing: fnl = (Z(D) + conj(ph) Z(Ds)) / 2, ph of the output mode
             const cpx<T>* ibs = (const cpx<T>*)a.Ds + (long long)cy * a.rs + kx;
+            const cpx<double> wcb = cmul(a.phc[kx], a.phb[cy]);
             has = terms(ibs);
             cpx<T> accs[I::RLAST];
 #pragma unroll
             for (int r = 0; r < I::RLAST; ++r) {
-                const cpx<double> q = a.ph[has ? j + r * I::BLAST : 0];
+                const cpx<double> q = cmul(a.ph[has ? j + r * I::BLAST : 0], wcb);
                 accs[r] = cmul(acc[r], cpx<T>{T(q.x), T(-q.y)});
             }
             has = terms(ib);

@@ -21,7 +21,11 @@ struct StridedArgs {
     long long in_b1, out_b1;
     int nb2;
     long long in_b2, out_b2;
-    const cpx<double>* ph;    // optional: input position p is multiplied by ph[p] (grid shift)
+    // optional grid shift (all three or none): the input at position p of
+    // column col in batch slab b1 is multiplied by ph[p] * phc[col] * phb[b1]
+    const cpx<double>* ph;
+    const cpx<double>* phc;
+    const cpx<double>* phb;
 };
 
 // Rows of an x transform: `nrows` rows (even), complex rows hold `kx` stored
@@ -100,8 +104,6 @@ struct XFArgs {
     void* out;                // flux spectral rows (T complex)
     int kx_o, rs_o;
     long long fs_o;
-    const cpx<double>* ph;    // optional grid shift: C2R input mode m times ph[m], R2C output mode k
-                              // times conj(ph[k])
 };
 template <typename T>
 hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp,
@@ -121,7 +123,6 @@ struct YGArgs {
     long long out_fs;
     int ngroups;
     YGroup g[8];
-    const cpx<double>* ph;    // optional grid shift: output mode p times conj(ph[p])
 };
 struct ZTerm {
     int field;
@@ -138,7 +139,9 @@ struct ZArgs {
     long long ufs;
     ZVar v[8];
     const void* Ds;           // optional second input (T complex, layout of D) from the shifted grid:
-    const cpx<double>* ph;    // fnl = (Z(D) + conj(ph[p]) Z(Ds)) / 2 at output mode p
+    const cpx<double>* ph;    // fnl = (Z(D) + conj(ph[p] phc[kx] phb[cy]) Z(Ds)) / 2 at output mode
+    const cpx<double>* phc;   // (kx, cy, p)
+    const cpx<double>* phb;
 };
 template <typename T>
 hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,

@@ -136,6 +136,7 @@ struct laps_ctx {
     void* P2 = nullptr;
     void* P3 = nullptr;                                   // shifted-grid y-pass output (option 3)
     cpx<double>*phx = nullptr, *phy = nullptr, *phz = nullptr;  // e^{i pi m/n} per axis (option 3)
+    cpx<double>* phyk = nullptr;                                 // phy of the kept box's stored ky
     cpx<double>* X1 = nullptr;
     cpx<double>* X2 = nullptr;
     double* Xr = nullptr;
@@ -335,14 +336,19 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 // (transform_uu_fourier_to_real, fftw.f90:73-103)
 // With dst == nullptr the x leg is left out and B holds the y-inverted rows.
 // With shift, every field is sampled on the grid shifted by half a cell along
-// each axis (phase-shift dealiasing): the z and y legs multiply their inputs
-// by e^{i pi m/n}; the x leg is the caller's.
+// each axis (phase-shift dealiasing): the z leg multiplies its input mode
+// (mx, my, mz) by e^{i pi (mx/nx + my/ny + mz/nz)}, the shift of all three
+// axes at once, as the modes are independent of each other in the y and x legs.
 template <typename T>
 int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
                  cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false) {
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
-    z.ph = shift ? c->phz : nullptr;
+    if (shift) {
+        z.ph = c->phz;
+        z.phc = c->phx;
+        z.phb = b.CY == c->ny ? c->phy : c->phyk;
+    }
     const long long RS = b.KXP;
     z.ncol = b.KX; z.in_cnt = b.CZ; z.out_cnt = (int)NZ;
     z.in_ls = (long long)b.CY * RS; z.out_ls = (long long)b.CY * RS;
@@ -352,7 +358,6 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
     LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + NZ * 2.0 * sizeof(T)),
                   [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
-    y.ph = shift ? c->phy : nullptr;
     y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
     y.in_ls = RS; y.out_ls = RS;
     y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * RS; y.out_b1 = NY * RS;
@@ -471,8 +476,8 @@ int substage(laps_ctx* c, int irk) {
     ZArgs za = aeb ? c->zr_aeb : c->zr;
     if (c->p.dealias_option == 3) {
         // phase-shift dealiasing: the same chain on the grid shifted by half a cell along every
-        // axis (inputs times e^{i pi m/n} per axis, outputs times the conjugate); its y-pass
-        // output goes to P3 and the z pass averages the two right-hand sides
+        // axis (z-inverse input times e^{i pi m/n} of all three axes, z-forward output times the
+        // conjugate); its y-pass output goes to P3 and the z pass averages the two right-hand sides
         cpx<T>* P3 = (cpx<T>*)c->P3;
         XFArgs xs = xa;
         if (first) {
@@ -494,19 +499,19 @@ int substage(laps_ctx* c, int irk) {
         xs.kx_j = xs.kx_s;
         xs.rs_j = xs.rs_s;
         xs.fs_j = xs.fs_s;
-        xs.ph = c->phx;
         const double xins = (double)c->nf_in() * xs.nrows * xs.kx_s * 2 * sizeof(T);
         LAPS_CK(timed(c, K_XFUSED, xins + xout, [&] {
             return launch_xfused<T>(c->nx, false, false, xs, tw_of<T>(c, 0), c->pp, c->ctl, c->stream);
         }));
-        YGArgs ys = ya;
-        ys.ph = c->phy;
+        const YGArgs& ys = ya;
         LAPS_CK(timed(c, K_YFWD, c->yfwd_bytes * sizeof(T), [&] {
             return launch_yfwd_group<T>(c->ny, P1, P3, ys, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
                                         c->nz, c->stream);
         }));
         za.Ds = P3;
         za.ph = c->phz;
+        za.phc = c->phx;
+        za.phb = c->phyk;
     }
     const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
     // U read + write (FP64), Fp write (+ read when irk > 0) in the transform type
@@ -865,10 +870,12 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     if (shift) {
         // half a cell along every axis: e^{i k delta} = e^{i pi m/n}, signed m with the +Nyquist
         // convention of the wave numbers
-        auto table = [&](cpx<double>** dst, int n, int cnt) -> int {
+        // (pruned: entry i is stored row i of an axis kept as cnt rows, 0..(cnt-1)/2 and then the
+        // negative modes up to n - 1)
+        auto table = [&](cpx<double>** dst, int n, int cnt, bool pruned) -> int {
             std::vector<cpx<double>> h(cnt);
             for (int i = 0; i < cnt; ++i) {
-                const int m = signed_index(i, n);
+                const int m = signed_index(pruned && i > (cnt - 1) / 2 ? n - (cnt - i) : i, n);
                 h[i].x = cos(kPiRef * m / n);
                 h[i].y = sin(kPiRef * m / n);
             }
@@ -878,8 +885,9 @@ int laps_create(const laps_params* p, laps_ctx** out) {
                 return fail(LAPS_EDEVICE, "phase table upload failed");
             return LAPS_OK;
         };
-        if ((rc = c->alloc(&c->P3, p2 * 2 * ts)) || (rc = table(&c->phx, c->nx, c->nxh)) ||
-            (rc = table(&c->phy, c->ny, c->ny)) || (rc = table(&c->phz, c->nz, c->nz)))
+        if ((rc = c->alloc(&c->P3, p2 * 2 * ts)) || (rc = table(&c->phx, c->nx, c->nxh, false)) ||
+            (rc = table(&c->phy, c->ny, c->ny, false)) || (rc = table(&c->phz, c->nz, c->nz, false)) ||
+            (rc = table(&c->phyk, c->ny, k.CY, true)))
             return bail(rc);
     }
     if ((rc = reset_ctl(c, 0.0, 0.0, false))) return bail(rc);

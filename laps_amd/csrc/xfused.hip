@@ -147,27 +147,12 @@ __device__ __forceinline__ void c2r_load(const cpx<T>* __restrict__ A0, const cp
 }
 
 // Z_p of the paired C2R (Z = A + iB with Hermitian extension; only the real
-// parts of the DC and Nyquist coefficients enter, as in FFTW's c2r). With a
-// grid shift, A and B at mode m are first multiplied by ph[m].
+// parts of the DC and Nyquist coefficients enter, as in FFTW's c2r).
 template <int N, typename T>
-__device__ __forceinline__ void c2r_combine(const cpx<T> (&a0)[4], const cpx<T> (&b0)[4], int kx, int lane,
-                                            const cpx<double>* ph, cpx<T> (&v)[4]) {
+__device__ __forceinline__ void c2r_combine(const cpx<T> (&a)[4], const cpx<T> (&b)[4], int kx, int lane,
+                                            cpx<T> (&v)[4]) {
     constexpr int LR = LaneFft<N>::LR;
     const cpx<T> zero{T(0), T(0)};
-    cpx<T> a[4], b[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        a[r] = a0[r];
-        b[r] = b0[r];
-        if (ph) {
-            const int p = lane + r * LR;
-            const int m = r < 2 ? p : N - p;
-            const cpx<double> w = ph[m < kx ? m : 0];
-            const cpx<T> wt{T(w.x), T(w.y)};
-            a[r] = cmul(a[r], wt);
-            b[r] = cmul(b[r], wt);
-        }
-    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int p = lane + r * LR;
@@ -182,11 +167,11 @@ __device__ __forceinline__ void c2r_combine(const cpx<T> (&a0)[4], const cpx<T> 
 
 // The two paired C2Rs of q at once: real rows (a0, b0) and (a1, b1).
 template <int N, typename T>
-__device__ __forceinline__ void c2r_pair2(const C2RRaw<T>& q, int kx, const cpx<double>* ph, cpx<T>* const (&line)[2], const cpx<T>* tw,
+__device__ __forceinline__ void c2r_pair2(const C2RRaw<T>& q, int kx, cpx<T>* const (&line)[2], const cpx<T>* tw,
                                           int lane, T (&a0)[4], T (&b0)[4], T (&a1)[4], T (&b1)[4]) {
     cpx<T> v[2][4];
-    c2r_combine<N, T>(q.a[0], q.b[0], kx, lane, ph, v[0]);
-    c2r_combine<N, T>(q.a[1], q.b[1], kx, lane, ph, v[1]);
+    c2r_combine<N, T>(q.a[0], q.b[0], kx, lane, v[0]);
+    c2r_combine<N, T>(q.a[1], q.b[1], kx, lane, v[1]);
     lane_fft<N, +1, 2, T>(v, line, tw, lane);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -213,7 +198,7 @@ __device__ __forceinline__ void r2c_mirror(const cpx<T> (&v)[4], cpx<T>* line, i
 
 template <int N, typename T>
 __device__ __forceinline__ void r2c_split(const cpx<T> (&v)[4], const cpx<T>* line, int lane, int kx, cpx<T>* oa,
-                                          cpx<T>* ob, const cpx<double>* ph) {
+                                          cpx<T>* ob) {
     using P = LanePad<T>;
     constexpr int LR = LaneFft<N>::LR;
     const T h = T(0.5);
@@ -223,15 +208,8 @@ __device__ __forceinline__ void r2c_split(const cpx<T> (&v)[4], const cpx<T>* li
         if (k < kx) {
             const cpx<T> zk = v[r];
             const cpx<T> zn = k == 0 ? zk : *(line + P::at(N - k));
-            cpx<T> A{h * (zk.x + zn.x), h * (zk.y - zn.y)};
-            cpx<T> B{h * (zk.y + zn.y), h * (zn.x - zk.x)};
-            if (ph) {  // back from the shifted grid: times conj(ph[k])
-                const cpx<T> w{T(ph[k].x), T(-ph[k].y)};
-                A = cmul(A, w);
-                B = cmul(B, w);
-            }
-            oa[k] = A;
-            if (ob) ob[k] = B;
+            oa[k] = cpx<T>{h * (zk.x + zn.x), h * (zk.y - zn.y)};
+            if (ob) ob[k] = cpx<T>{h * (zk.y + zn.y), h * (zn.x - zk.x)};
         }
     }
 }
@@ -239,23 +217,21 @@ __device__ __forceinline__ void r2c_split(const cpx<T> (&v)[4], const cpx<T>* li
 // forward transform of one pair (fa, fb) -> (oa, ob)
 template <int N, typename T>
 __device__ __forceinline__ void r2c_pair(const T (&fa)[4], const T (&fb)[4], cpx<T>* oa, cpx<T>* ob, int kx,
-                                         cpx<T>* const (&line)[1], const cpx<T>* tw, int lane,
-                                         const cpx<double>* ph) {
+                                         cpx<T>* const (&line)[1], const cpx<T>* tw, int lane) {
     cpx<T> v[1][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) v[0][r] = cpx<T>{fa[r], fb[r]};
     lane_fft<N, -1, 1, T>(v, line, tw, lane);
     r2c_mirror<N, T>(v[0], line[0], lane);
     __syncthreads();
-    r2c_split<N, T>(v[0], line[0], lane, kx, oa, ob, ph);
+    r2c_split<N, T>(v[0], line[0], lane, kx, oa, ob);
 }
 
 // two forward transforms at once: (fa0, fb0) -> (oa0, ob0), (fa1, fb1) -> (oa1, ob1)
 template <int N, typename T>
 __device__ __forceinline__ void r2c_pair2(const T (&fa0)[4], const T (&fb0)[4], cpx<T>* oa0, cpx<T>* ob0,
                                           const T (&fa1)[4], const T (&fb1)[4], cpx<T>* oa1, cpx<T>* ob1, int kx,
-                                          cpx<T>* const (&line)[2], const cpx<T>* tw, int lane,
-                                          const cpx<double>* ph) {
+                                          cpx<T>* const (&line)[2], const cpx<T>* tw, int lane) {
     cpx<T> v[2][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -266,8 +242,8 @@ __device__ __forceinline__ void r2c_pair2(const T (&fa0)[4], const T (&fb0)[4], 
     r2c_mirror<N, T>(v[0], line[0], lane);
     r2c_mirror<N, T>(v[1], line[1], lane);
     __syncthreads();
-    r2c_split<N, T>(v[0], line[0], lane, kx, oa0, ob0, ph);
-    r2c_split<N, T>(v[1], line[1], lane, kx, oa1, ob1, ph);
+    r2c_split<N, T>(v[0], line[0], lane, kx, oa0, ob0);
+    r2c_split<N, T>(v[1], line[1], lane, kx, oa1, ob1);
 }
 
 template <typename T, int N, bool FIRST, bool CFL, bool HALL, bool AEB>
@@ -322,9 +298,9 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
         c2r_load<N, true, T>(st, st + fs, st + 2 * fs, st + 3 * fs, a.kx_s, lane, qa);
         stage_twiddles();
         c2r_load<N, true, T>(st + 4 * fs, st + 5 * fs, st + 6 * fs, st + 7 * fs, a.kx_s, lane, qb);
-        c2r_pair2<N, T>(qa, a.kx_s, a.ph, line2, tw, lane, rho, mx, my, mz);
+        c2r_pair2<N, T>(qa, a.kx_s, line2, tw, lane, rho, mx, my, mz);
         if constexpr (HALL) c2r_load<N, false, T>(jb, jb + fsj, jb + 2 * fsj, nullptr, a.kx_j, lane, qa);
-        c2r_pair2<N, T>(qb, a.kx_s, a.ph, line2, tw, lane, bx, by, bz, e);
+        c2r_pair2<N, T>(qb, a.kx_s, line2, tw, lane, bx, by, bz, e);
         const T g1 = T(pp.gamma) - T(1);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -356,7 +332,7 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
 
     if constexpr (HALL) {
         T dummy[4];
-        c2r_pair2<N, T>(qa, a.kx_j, a.ph, line2, tw, lane, jx, jy, jz, dummy);
+        c2r_pair2<N, T>(qa, a.kx_j, line2, tw, lane, jx, jy, jz, dummy);
     }
 
     // ---- fluxes (mhdrhs.f90:60-119), kept as point values
@@ -397,28 +373,28 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     // (E_x, E_y) with (E_z, energy flux x)
 #pragma unroll
     for (int r = 0; r < 4; ++r) { f0[r] = ez[r]; f1[r] = ep[r] * ux[r] - udb[r] * bx[r]; }
-    r2c_pair2<N, T>(ex, ey, O(9), O(10), f0, f1, O(11), O(12), kxo, line2, tw, lane, a.ph);
+    r2c_pair2<N, T>(ex, ey, O(9), O(10), f0, f1, O(11), O(12), kxo, line2, tw, lane);
     // (energy flux y, z) with (M_xx, M_xy)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         f0[r] = ep[r] * uy[r] - udb[r] * by[r]; f1[r] = ep[r] * uz[r] - udb[r] * bz[r];
         f2[r] = mx[r] * ux[r] - bx[r] * bx[r] + ptot[r]; f3[r] = my[r] * ux[r] - by[r] * bx[r];
     }
-    r2c_pair2<N, T>(f0, f1, O(13), O(14), f2, f3, O(3), O(4), kxo, line2, tw, lane, a.ph);
+    r2c_pair2<N, T>(f0, f1, O(13), O(14), f2, f3, O(3), O(4), kxo, line2, tw, lane);
     // (M_xz, M_yy) with (M_yz, M_zz)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         f0[r] = mz[r] * ux[r] - bz[r] * bx[r]; f1[r] = my[r] * uy[r] - by[r] * by[r] + ptot[r];
         f2[r] = mz[r] * uy[r] - bz[r] * by[r]; f3[r] = mz[r] * uz[r] - bz[r] * bz[r] + ptot[r];
     }
-    r2c_pair2<N, T>(f0, f1, O(5), O(6), f2, f3, O(7), O(8), kxo, line2, tw, lane, a.ph);
+    r2c_pair2<N, T>(f0, f1, O(5), O(6), f2, f3, O(7), O(8), kxo, line2, tw, lane);
     // (mass flux x, y) with (mass flux z, expanding-box source or nothing)
     if constexpr (AEB) {
-        r2c_pair2<N, T>(mx, my, O(0), O(1), mz, f19, O(2), O(15), kxo, line2, tw, lane, a.ph);
+        r2c_pair2<N, T>(mx, my, O(0), O(1), mz, f19, O(2), O(15), kxo, line2, tw, lane);
     } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) f0[r] = T(0);
-        r2c_pair2<N, T>(mx, my, O(0), O(1), mz, f0, O(2), nullptr, kxo, line2, tw, lane, a.ph);
+        r2c_pair2<N, T>(mx, my, O(0), O(1), mz, f0, O(2), nullptr, kxo, line2, tw, lane);
     }
 }
 
