@@ -18,7 +18,7 @@ namespace laps {
 
 template <typename T, int N, class P = Plan<N>>
 struct StridedCfg {
-    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (N >= 512 ? 8 : 16);
+    static constexpr int TX = (N >= 512) ? (sizeof(T) == 8 ? 4 : 8) : 16;  // 256-byte FP64 rows
     static constexpr int TPL = PlanInfo<N, P>::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;  // odd line stride: adjacent lines hit different banks
@@ -28,15 +28,33 @@ template <typename TI, typename T, int N, int DIR, class P>
 __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx<TI>* __restrict__ in,
                                                                   cpx<T>* __restrict__ out, StridedArgs a,
                                                                   const cpx<T>* __restrict__ tw, T scale,
-                                                                  int ntile) {
+                                                                  int ntile, long long nrest) {
     using C = StridedCfg<T, N, P>;
     __shared__ cpx<T> sh[C::TX * C::LS];
     __shared__ cpx<T> stw[N];
+    long long bid = blockIdx.x;
+    const cpx<double>* ph = a.ph;
+    if (a.nch == 2) {
+        int ch;
+        if (a.pair) {
+            // blocks b and b + 8 run the same slab of chains 0 and 1: work-groups are dealt
+            // round-robin over the 8 XCDs, so both land on one XCD and the second reads the
+            // input through that XCD's L2. The grid is padded to whole groups of 16.
+            ch = (int)((bid >> 3) & 1);
+            bid = ((bid >> 4) << 3) | (bid & 7);
+            if (bid >= nrest) return;  // whole work-group, before any barrier
+        } else {
+            ch = bid >= nrest ? 1 : 0;  // the blocks of chain 1 follow those of chain 0
+            bid -= ch * nrest;
+        }
+        in += ch * a.in_ch;
+        out += ch * a.out_ch;
+        if (ch == 0) ph = nullptr;
+    }
     LdsTable<cpx<T>, N, C::NT> twt;
     twt.fetch(tw);
     const int t = threadIdx.x;
     const int l = t % C::TX, j = t / C::TX;
-    long long bid = blockIdx.x;
     const int tile = (int)(bid % ntile);
     bid /= ntile;
     const int b1 = (int)(bid % a.nb1);
@@ -48,7 +66,6 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
     cpx<T>* line = sh + l * C::LS;
     const int in_cnt = a.in_cnt, out_cnt = a.out_cnt;
     const long long in_ls = a.in_ls, out_ls = a.out_ls;
-    const cpx<double>* ph = a.ph;
     // grid shift: the column and slab factors are fixed per work-item
     const cpx<double> wcb = ph ? cmul(a.phc[valid ? col : 0], a.phb[b1]) : cpx<double>{1.0, 0.0};
     auto load = [&](int p) -> cpx<T> {
@@ -173,10 +190,11 @@ static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs
                                hipStream_t s) {
     using C = StridedCfg<T, N, P>;
     const int ntile = (a.ncol + C::TX - 1) / C::TX;
-    const long long nblk = (long long)ntile * a.nb1 * a.nb2;
-    if (nblk <= 0) return hipSuccess;
+    const long long nrest = (long long)ntile * a.nb1 * a.nb2;
+    if (nrest <= 0) return hipSuccess;
+    const long long nblk = a.nch == 2 ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest;
     hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, scale,
-                       ntile);
+                       ntile, nrest);
     return hipGetLastError();
 }
 

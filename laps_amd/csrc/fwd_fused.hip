@@ -20,12 +20,14 @@
 
 namespace laps {
 
-template <typename T, int N>
+// ZP: the z pass, whose blocks also carry the RHS and RK3 update
+template <typename T, int N, bool ZP>
 struct FwdCfg {
     using P = PlanLow<N>;
     using I = PlanInfo<N, P>;
-    // columns per block: 8 for FP32 as well keeps blocks at 8 waves, so several fit on a CU
-    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : 8;
+    // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns) in the y pass; the z
+    // pass keeps 8 FP32 columns, whose blocks of 8 waves leave room for more per CU
+    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (ZP || N >= 512 ? 8 : 16);
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
@@ -47,13 +49,13 @@ __device__ __forceinline__ cpx<T> mul_i(T k, cpx<T> z) { return {-k * z.y, k * z
 
 // ---------------------------------------------------------------- y pass
 template <typename T, int N>
-__global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>* __restrict__ in,
+__global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const cpx<T>* __restrict__ in,
                                                                  cpx<T>* __restrict__ out, YGArgs a,
                                                                  const cpx<T>* __restrict__ tw,
                                                                  const double* __restrict__ wxt,
                                                                  const double* __restrict__ wyf, PhysParams pp,
                                                                  const Ctl* __restrict__ ctl, int ntile, int nz) {
-    using C = FwdCfg<T, N>;
+    using C = FwdCfg<T, N, false>;
     using I = typename C::I;
     __shared__ cpx<T> sh[C::TX * C::LS];
     __shared__ cpx<T> stw[N];
@@ -66,6 +68,12 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
     wyt.put(swy);
     const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
     long long bid = blockIdx.x;
+    const long long bpc = (long long)ntile * nz * a.ngroups;  // blocks per chain
+    if (a.nch == 2 && bid >= bpc) {
+        bid -= bpc;
+        in += a.in_ch;
+        out += a.out_ch;
+    }
     const int tile = (int)(bid % ntile);
     bid /= ntile;
     const int z = (int)(bid % nz);
@@ -196,14 +204,14 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_yfwd_group(const cpx<T>*
 // ---------------------------------------------------------------- z pass + RK3
 // Output variable v is  sum_t  m_t * Z(D[f_t])  with m_t in {1, -i kz', +i kz'}.
 template <typename T, int N, int OPT>
-__global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* __restrict__ D,
+__global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx<T>* __restrict__ D,
                                                                cpx<double>* __restrict__ U,
                                                                cpx<T>* __restrict__ Fp, ZArgs a,
                                                                const cpx<T>* __restrict__ tw, BoxTables tb,
                                                                const double* __restrict__ wzf, PhysParams pp,
                                                                const Ctl* __restrict__ ctl, int irk, int ntile,
                                                                T scale) {
-    using C = FwdCfg<T, N>;
+    using C = FwdCfg<T, N, true>;
     using I = typename C::I;
     if (ctl->status != 0) return;  // uniform over the grid: the step was refused
     __shared__ cpx<T> sh[C::TX * C::LS];
@@ -347,9 +355,9 @@ __global__ __launch_bounds__((FwdCfg<T, N>::NT)) void k_zfwd_rhs(const cpx<T>* _
 template <typename T, int N>
 static hipError_t do_yg(const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw, const double* wxt,
                         const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz, hipStream_t s) {
-    using C = FwdCfg<T, N>;
+    using C = FwdCfg<T, N, false>;
     const int ntile = (a.ncol + C::TX - 1) / C::TX;
-    const long long nblk = (long long)ntile * nz * a.ngroups;
+    const long long nblk = (long long)ntile * nz * a.ngroups * (a.nch == 2 ? 2 : 1);
     hipLaunchKernelGGL((k_yfwd_group<T, N>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, wxt, wyf, pp,
                        ctl, ntile, nz);
     return hipGetLastError();
@@ -374,7 +382,7 @@ template <typename T, int N, int OPT>
 static hipError_t do_zr(const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, const ZArgs& a, const cpx<T>* tw,
                         const BoxTables& tb, const double* wzf, const PhysParams& pp, const Ctl* ctl, int irk,
                         T scale, hipStream_t s) {
-    using C = FwdCfg<T, N>;
+    using C = FwdCfg<T, N, true>;
     const int ntile = (a.KX + C::TX - 1) / C::TX;
     const long long nblk = 8LL * ntile * a.CY;
     hipLaunchKernelGGL((k_zfwd_rhs<T, N, OPT>), dim3((unsigned)nblk), dim3(C::NT), 0, s, D, U, Fp, a, tw, tb, wzf, pp,

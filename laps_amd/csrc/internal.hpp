@@ -22,10 +22,17 @@ struct StridedArgs {
     int nb2;
     long long in_b2, out_b2;
     // optional grid shift (all three or none): the input at position p of
-    // column col in batch slab b1 is multiplied by ph[p] * phc[col] * phb[b1]
+    // column col in batch slab b1 is multiplied by ph[p] * phc[col] * phb[b1];
+    // with two chains only chain 1 is shifted
     const cpx<double>* ph;
     const cpx<double>* phc;
     const cpx<double>* phb;
+    // nch = 2: a second chain of the same shape at in + in_ch, out + out_ch in
+    // the same launch (0 or 1 counts as one chain); pair: chain 0 and chain 1 of a
+    // slab run side by side on one XCD (for inputs the two chains share)
+    int nch;
+    long long in_ch, out_ch;
+    bool pair;
 };
 
 // Rows of an x transform: `nrows` rows (even), complex rows hold `kx` stored
@@ -104,6 +111,10 @@ struct XFArgs {
     void* out;                // flux spectral rows (T complex)
     int kx_o, rs_o;
     long long fs_o;
+    // nch = 2: a second chain of rows at st + st_ch, jb + jb_ch, out + out_ch (elements) in the
+    // same launch, without the CFL reduction (0 or 1 counts as one chain)
+    int nch;
+    long long st_ch, jb_ch, out_ch;
 };
 template <typename T>
 hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp,
@@ -123,6 +134,9 @@ struct YGArgs {
     long long out_fs;
     int ngroups;
     YGroup g[8];
+    // nch = 2: a second chain at in + in_ch, out + out_ch in the same launch
+    int nch;
+    long long in_ch, out_ch;
 };
 struct ZTerm {
     int field;

@@ -134,7 +134,8 @@ struct laps_ctx {
     double* S = nullptr;
     void* P1 = nullptr;
     void* P2 = nullptr;
-    void* P3 = nullptr;                                   // shifted-grid y-pass output (option 3)
+    void* P3 = nullptr;                                   // shifted-grid y-pass output (option 3), in P2
+    long long p1 = 0, p2 = 0;                             // complex elements per chain of P1, P2
     cpx<double>*phx = nullptr, *phy = nullptr, *phz = nullptr;  // e^{i pi m/n} per axis (option 3)
     cpx<double>* phyk = nullptr;                                 // phy of the kept box's stored ky
     cpx<double>* X1 = nullptr;
@@ -339,9 +340,13 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 // each axis (phase-shift dealiasing): the z leg multiplies its input mode
 // (mx, my, mz) by e^{i pi (mx/nx + my/ny + mz/nz)}, the shift of all three
 // axes at once, as the modes are independent of each other in the y and x legs.
+// With nch = 2 the z leg also writes the shifted fields as chain 1 (at A + a_ch),
+// read once from src, and the y leg transforms both chains (chain 1 to B + b_ch).
 template <typename T>
 int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
-                 cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false) {
+                 cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false, int nch = 1, long long a_ch = 0,
+                 long long b_ch = 0) {
+    if (nch == 2 && (dst || !shift)) return fail(LAPS_EINVAL, "internal: two chains are the shifted pair");
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
     if (shift) {
@@ -354,16 +359,18 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
     z.in_ls = (long long)b.CY * RS; z.out_ls = (long long)b.CY * RS;
     z.nb1 = b.CY; z.in_b1 = RS; z.out_b1 = RS;
     z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * RS;
+    z.nch = nch; z.out_ch = a_ch; z.pair = true;
     const double ez = (double)nf * b.CY * b.KX;
-    LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + NZ * 2.0 * sizeof(T)),
+    LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + nch * NZ * 2.0 * sizeof(T)),
                   [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
     y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
     y.in_ls = RS; y.out_ls = RS;
     y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * RS; y.out_b1 = NY * RS;
     y.nb2 = nf; y.in_b2 = NZ * b.CY * RS; y.out_b2 = NZ * NY * RS;
+    y.nch = nch; y.in_ch = a_ch; y.out_ch = b_ch;
     const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
-    LAPS_CK(timed(c, K_YINV, ey * (b.CY + NY),
+    LAPS_CK(timed(c, K_YINV, nch * ey * (b.CY + NY),
                   [&] { return launch_strided<T, T>(c->ny, +1, A, B, y, tw_of<T>(c, 1), T(1), c->stream, plan_y); }));
     if (!dst) return LAPS_OK;
     XArgs x{};
@@ -427,6 +434,9 @@ int substage(laps_ctx* c, int irk) {
     xa.rs_o = k.KXP;
     xa.fs_o = NZ * NY * k.KXP;
     double xin;
+    // phase-shift dealiasing after the first substage: the plain and the shifted grid run as
+    // chains 0 and 1 of each launch of the inverse, x and y-forward passes
+    const int nch = (!first && c->p.dealias_option == 3) ? 2 : 1;
     if (first) {
         // first substage: the uploaded real fields are used as they are; J
         // comes from the un-dealiased spectrum (mhd.f90:121-122, :305)
@@ -447,8 +457,13 @@ int substage(laps_ctx* c, int irk) {
             LAPS_CK(timed(c, K_CURL, 96.0 * KB, [&] {
                 return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
             }));
-        if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv, c->plan_yinv)))
+        if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv, c->plan_yinv,
+                                  nch == 2, nch, c->p1, c->p2)))
             return rc;
+        xa.nch = nch;
+        xa.st_ch = c->p2;
+        xa.jb_ch = c->p2;
+        xa.out_ch = c->p1;
         xa.st = P2;
         xa.kx_s = k.KX;
         xa.rs_s = k.KXP;
@@ -457,43 +472,37 @@ int substage(laps_ctx* c, int irk) {
         xa.kx_j = k.KX;
         xa.rs_j = k.KXP;
         xa.fs_j = xa.fs_s;
-        xin = (double)c->nf_in() * xa.nrows * k.KX * 2 * sizeof(T);
+        xin = (double)nch * c->nf_in() * xa.nrows * k.KX * 2 * sizeof(T);
     }
     const bool cfl = irk == 0 && !(c->fixed_dt > 0);
     if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_begin(c->ctl, c->stream); }));
     const double xout = (double)c->nf_out() * xa.nrows * k.KX * 2 * sizeof(T);
-    LAPS_CK(timed(c, K_XFUSED, xin + xout, [&] {
+    LAPS_CK(timed(c, K_XFUSED, xin + nch * xout, [&] {
         return launch_xfused<T>(c->nx, first, cfl, xa, tw_of<T>(c, 0), c->pp, c->ctl, c->stream);
     }));
     if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_finalize(c->ctl, c->pp, c->stream); }));
     // y forward with the x/y derivative terms folded in, then z forward + RHS + RK3 + dealias
     const bool aeb = c->p.if_aeb != 0;
-    const YGArgs& ya = aeb ? c->yg_aeb : c->yg;
-    LAPS_CK(timed(c, K_YFWD, c->yfwd_bytes * sizeof(T), [&] {
+    YGArgs ya = aeb ? c->yg_aeb : c->yg;
+    ya.nch = nch;
+    ya.in_ch = c->p1;
+    ya.out_ch = c->p2;
+    LAPS_CK(timed(c, K_YFWD, nch * c->yfwd_bytes * sizeof(T), [&] {
         return launch_yfwd_group<T>(c->ny, P1, P2, ya, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
                                     c->nz, c->stream);
     }));
     ZArgs za = aeb ? c->zr_aeb : c->zr;
-    if (c->p.dealias_option == 3) {
-        // phase-shift dealiasing: the same chain on the grid shifted by half a cell along every
-        // axis (z-inverse input times e^{i pi m/n} of all three axes, z-forward output times the
-        // conjugate); its y-pass output goes to P3 and the z pass averages the two right-hand sides
-        cpx<T>* P3 = (cpx<T>*)c->P3;
+    cpx<T>* P3 = (cpx<T>*)c->P3;
+    if (first && c->p.dealias_option == 3) {
+        // phase-shift dealiasing, first substage: the shifted grid from the full spectrum, after
+        // the plain one (z-inverse input times e^{i pi m/n} of all three axes, z-forward output
+        // times the conjugate); its y-pass output goes to P3
         XFArgs xs = xa;
-        if (first) {
-            if ((rc = inv_pipeline<T>(c, c->Ufull, c->M, c->full, c->nf_in(), (T*)nullptr, P1, P3, 0, 0, true)))
-                return rc;
-            xs.kx_s = c->nxh;
-            xs.rs_s = c->full.KXP;
-            xs.fs_s = NZ * NY * c->full.KXP;
-        } else {
-            if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P3, c->plan_zinv, c->plan_yinv,
-                                      true)))
-                return rc;
-            xs.kx_s = k.KX;
-            xs.rs_s = k.KXP;
-            xs.fs_s = NZ * NY * k.KXP;
-        }
+        if ((rc = inv_pipeline<T>(c, c->Ufull, c->M, c->full, c->nf_in(), (T*)nullptr, P1, P3, 0, 0, true)))
+            return rc;
+        xs.kx_s = c->nxh;
+        xs.rs_s = c->full.KXP;
+        xs.fs_s = NZ * NY * c->full.KXP;
         xs.st = P3;
         xs.jb = P3 + 8 * xs.fs_s;
         xs.kx_j = xs.kx_s;
@@ -503,11 +512,13 @@ int substage(laps_ctx* c, int irk) {
         LAPS_CK(timed(c, K_XFUSED, xins + xout, [&] {
             return launch_xfused<T>(c->nx, false, false, xs, tw_of<T>(c, 0), c->pp, c->ctl, c->stream);
         }));
-        const YGArgs& ys = ya;
         LAPS_CK(timed(c, K_YFWD, c->yfwd_bytes * sizeof(T), [&] {
-            return launch_yfwd_group<T>(c->ny, P1, P3, ys, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
+            return launch_yfwd_group<T>(c->ny, P1, P3, ya, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
                                         c->nz, c->stream);
         }));
+    }
+    if (c->p.dealias_option == 3) {
+        // the z pass averages the right-hand sides of the two grids
         za.Ds = P3;
         za.ph = c->phz;
         za.phc = c->phx;
@@ -857,12 +868,25 @@ int laps_create(const laps_params* p, laps_ctx** out) {
         p2 = std::max(p2, (long long)nfi * NZ * NY * c->full.KXP);
     }
     const size_t ts = c->mixed ? sizeof(float) : sizeof(double);
+    // phase-shift dealiasing runs the plain and the shifted grid as two chains of one launch
+    // per pass: P1 and P2 hold both, chain 1 at + p1 and + p2 (P3 = P2 + p2)
+    // (chain 1 starts 5 KiB past a whole number of 64 KiB, so that the two chains' rows at the
+    // same offset do not fall on the same HBM channels)
+    auto chain_pad = [&](long long e) {
+        const long long unit = 65536 / (2 * (long long)ts), pad = 5120 / (2 * (long long)ts);
+        return (e + unit - 1) / unit * unit + pad;
+    };
+    p1 = chain_pad(p1);
+    p2 = chain_pad(p2);
+    c->p1 = p1;
+    c->p2 = p2;
+    const int nchain = shift ? 2 : 1;
     // FP64 single-field scratch: a full spectrum, or one kept-box field inverted along z and y
     const size_t xs = (size_t)std::max(c->M, NZ * NY * (long long)k.KXP);
     (void)NX;
     if ((rc = c->allocn(&c->ctl, 1)) || (rc = c->allocn(&c->U, 11 * KB)) || (rc = c->allocn(&c->Ufull, 11 * c->M)) ||
         (rc = c->allocn(&c->Fp, 8 * KB)) || (rc = c->allocn(&c->S, 12 * c->N)) ||
-        (rc = c->alloc(&c->P1, p1 * 2 * ts)) || (rc = c->alloc(&c->P2, p2 * 2 * ts)) ||
+        (rc = c->alloc(&c->P1, nchain * p1 * 2 * ts)) || (rc = c->alloc(&c->P2, nchain * p2 * 2 * ts)) ||
         (rc = c->allocn(&c->X1, xs)) || (rc = c->allocn(&c->X2, xs)) || (rc = c->allocn(&c->Xr, c->N)) ||
         (rc = c->allocn(&c->red, (size_t)c->nblk_red * 16)) || (rc = c->allocn(&c->dave, 8)) ||
         (rc = c->allocn(&c->dbits, 1)))
@@ -885,7 +909,8 @@ int laps_create(const laps_params* p, laps_ctx** out) {
                 return fail(LAPS_EDEVICE, "phase table upload failed");
             return LAPS_OK;
         };
-        if ((rc = c->alloc(&c->P3, p2 * 2 * ts)) || (rc = table(&c->phx, c->nx, c->nxh, false)) ||
+        c->P3 = (char*)c->P2 + p2 * 2 * ts;
+        if ((rc = table(&c->phx, c->nx, c->nxh, false)) ||
             (rc = table(&c->phy, c->ny, c->ny, false)) || (rc = table(&c->phz, c->nz, c->nz, false)) ||
             (rc = table(&c->phyk, c->ny, k.CY, true)))
             return bail(rc);

@@ -258,7 +258,12 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     const cpx<T>* tw = stw;
     const int lane = threadIdx.x % LR;
     const int rib = threadIdx.x / LR;
-    long long row = (long long)blockIdx.x * RPB + rib;
+    // two chains: the blocks of chain 1 follow those of chain 0
+    long long blk = blockIdx.x;
+    const long long gpc = (a.nrows + RPB - 1) / RPB;
+    const int ch = (a.nch == 2 && blk >= gpc) ? 1 : 0;
+    blk -= ch * gpc;
+    long long row = blk * RPB + rib;
     const bool valid = row < a.nrows;
     if (!valid) row = a.nrows - 1;  // computes a duplicate row, never stored
     cpx<T>* const line1[1] = {sh + 2 * rib * LS};
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     T rho[4], mx[4], my[4], mz[4], bx[4], by[4], bz[4], e[4], jx[4], jy[4], jz[4];
     T ux[4], uy[4], uz[4], p[4];
     C2RRaw<T> qa, qb;
-    const cpx<T>* jb = (const cpx<T>*)a.jb + row * a.rs_j;
+    const cpx<T>* jb = (const cpx<T>*)a.jb + ch * a.jb_ch + row * a.rs_j;
     const long long fsj = a.fs_j;
     if constexpr (FIRST) {
         const long long N3 = a.npts;
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
         if constexpr (HALL) c2r_load<N, false, T>(jb, jb + fsj, jb + 2 * fsj, nullptr, a.kx_j, lane, qa);
         stage_twiddles();
     } else {
-        const cpx<T>* st = (const cpx<T>*)a.st + row * a.rs_s;
+        const cpx<T>* st = (const cpx<T>*)a.st + ch * a.st_ch + row * a.rs_s;
         const long long fs = a.fs_s;
         c2r_load<N, true, T>(st, st + fs, st + 2 * fs, st + 3 * fs, a.kx_s, lane, qa);
         stage_twiddles();
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     }
     // ---- CFL minimum of the first substage (vardt, mhd.f90:352-416), in the
     // arithmetic of the pass (FP64, or FP32 in the mixed-precision step)
-    if constexpr (CFL) {
+    if (CFL && ch == 0) {
         const T rr = T(ctl->radius / pp.radius0);
         double dmin = pp.dx < pp.dy ? pp.dx : pp.dy;
         if (pp.dz < dmin) dmin = pp.dz;
@@ -358,7 +363,7 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
         }
     }
 
-    cpx<T>* out = (cpx<T>*)a.out + row * a.rs_o;
+    cpx<T>* out = (cpx<T>*)a.out + ch * a.out_ch + row * a.rs_o;
     const long long fo = a.fs_o;
     const int kxo = valid ? a.kx_o : 0;
     // Output slots: 0-2 mass flux (rho u), 3-8 the symmetric momentum flux tensor
@@ -402,7 +407,7 @@ template <typename T, int N, bool FIRST, bool CFL, bool HALL>
 static void xf_aeb(bool aeb, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp, Ctl* ctl, hipStream_t s) {
     constexpr int NT = LaneFft<N>::NT;
     constexpr int RPB = NT / LaneFft<N>::LR;
-    const unsigned g = (unsigned)((a.nrows + RPB - 1) / RPB);
+    const unsigned g = (unsigned)((a.nrows + RPB - 1) / RPB * (a.nch == 2 ? 2 : 1));
     if (aeb)
         hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, true>), dim3(g), dim3(NT), 0, s, a, tw, pp, ctl);
     else
