@@ -229,7 +229,7 @@ def main():
                     free_running=dict(rel_l2_vs_fp64=rel_free, time_fp64=f["scalars"]["time"],
                                       time_mixed=m["scalars"]["time"]),
                     roofline=roofline(m["timing"], "mixed" if dealias == 1 else "mixed_ps"),
-                    ms_per_step=round(1000.0 * m["seconds"] / args.steps, 4))
+                    ms_per_step=round(1000.0 * m["seconds"] / args.steps, 4), kernels=kernels(m))
 
     def kernels(r):
         return {k: dict(ms_per_step=round(v[0] / args.steps, 4), gbs=round(v[1] / (v[0] * 1e-3) / 1e9, 1)
