@@ -174,6 +174,28 @@ int laps_timing_read(laps_ctx* ctx, int index, const char** name, double* total_
  * own work after laps_step. No counterpart in the reference. */
 int laps_get_stream(laps_ctx* ctx, void** stream);
 
+/* Ensembles of independent boxes. No counterpart in the reference, which runs
+ * one box per MPI job (mhd.f90:56-167); callers: parameter scans and
+ * ensemble statistics, many small boxes on one GPU.
+ *
+ * laps_ens_create makes nbox contexts from params[0..nbox-1] (any mix of
+ * grids, physics switches and precisions; one device). laps_ens_box returns
+ * box i, a context owned by the ensemble: every per-context call above
+ * (laps_set_state, laps_get_*, diagnostics, laps_set_fixed_dt) applies to it,
+ * and each box keeps its own CFL dt, time and stop status. laps_ens_step
+ * advances every box by nsteps RK3 steps: a first step after any box's
+ * laps_set_state runs per box, then one step of all boxes is captured as a
+ * hipGraph (boxes as parallel branches) and every further step is one
+ * launch of it. Like laps_step it is asynchronous; per-box reads
+ * synchronise with it. laps_ens_destroy destroys the boxes too. */
+typedef struct laps_ens laps_ens;
+int laps_ens_create(int nbox, const laps_params* params, laps_ens** ens);
+void laps_ens_destroy(laps_ens* ens);
+int laps_ens_size(const laps_ens* ens);
+laps_ctx* laps_ens_box(laps_ens* ens, int i);
+int laps_ens_step(laps_ens* ens, int nsteps);
+int laps_ens_sync(laps_ens* ens);
+
 #ifdef __cplusplus
 }
 #endif

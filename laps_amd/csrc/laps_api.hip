@@ -20,6 +20,7 @@
 // transforms read and write only the box.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -146,6 +147,7 @@ struct laps_ctx {
     unsigned long long* dbits = nullptr;
     std::vector<void*> allocs;
     bool have_state = false, fresh = false, materialized = false, stepped_since_set = false;
+    uint64_t gen = 0;  // bumped when the kernels a step launches change (state upload, fixed dt)
     bool tuned = false;
     int plan_zinv = 0, plan_yinv = 0;  // FFT plans of the inverse z / y passes, chosen by autotune()
     YGArgs yg{}, yg_aeb{};
@@ -538,6 +540,15 @@ int substage(laps_ctx* c, int irk) {
 // (events on the context stream, on the real buffers) and keeps the faster.
 template <typename T>
 int autotune(laps_ctx* c) {
+    // LAPS_FFT_PLAN = 0 or 1 fixes both plans instead (reproducible runs: the timed choice
+    // can differ between contexts, and with it the rounding of the transforms)
+    if (const char* fixed = std::getenv("LAPS_FFT_PLAN")) {
+        if ((fixed[0] == '0' || fixed[0] == '1') && fixed[1] == 0) {
+            c->plan_zinv = c->plan_yinv = fixed[0] - '0';
+            c->tuned = true;
+            return LAPS_OK;
+        }
+    }
     TimingOff off(c);
     const Box& k = c->kept;
     cpx<T>* P1 = (cpx<T>*)c->P1;
@@ -946,6 +957,7 @@ int laps_set_state(laps_ctx* c, const double* uu, const double* uu_prim) {
     c->fresh = true;
     c->materialized = true;
     c->stepped_since_set = false;
+    ++c->gen;
     return LAPS_OK;
 }
 
@@ -969,6 +981,7 @@ int laps_set_fixed_dt(laps_ctx* c, double dt) {
     if (rc) return rc;
     c->fixed_dt = dt > 0 ? dt : 0.0;
     c->pp.fixed_dt = c->fixed_dt;
+    ++c->gen;
     return LAPS_OK;
 }
 
@@ -1216,6 +1229,168 @@ int laps_get_stream(laps_ctx* c, void** stream) {
     if (rc) return rc;
     if (!stream) return fail(LAPS_EINVAL, "null output");
     *stream = (void*)c->stream;
+    return LAPS_OK;
+}
+
+// ------------------------------------------------------------------ ensemble
+//
+// Independent boxes (each a full context with its own stream, buffers and
+// device-side dt) advanced together. The first step after a state upload runs
+// eagerly per box; after that one step of every box is captured once as a
+// hipGraph, the boxes as parallel branches forked from and joined into the
+// ensemble's own stream, and each further step is one launch of that graph.
+// A box whose launches change (new state, fixed dt) makes the next
+// laps_ens_step capture again.
+struct laps_ens {
+    std::vector<laps_ctx*> box;
+    std::vector<uint64_t> gen;         // box generations the graph was captured at
+    int device = 0;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr, after = nullptr;  // capture fork; order after launches
+    std::vector<hipEvent_t> join, before;        // capture joins; order before launches
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void ens_drop_graph(laps_ens* e) {
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+static int ens_capture(laps_ens* e) {
+    ens_drop_graph(e);
+    for (laps_ctx* c : e->box) c->timing.on = false;
+    LAPS_CK(hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
+    int rc = LAPS_OK;
+    hipError_t he = hipEventRecord(e->fork, e->origin);
+    for (size_t i = 0; i < e->box.size() && he == hipSuccess && rc == LAPS_OK; ++i) {
+        laps_ctx* c = e->box[i];
+        he = hipStreamWaitEvent(c->stream, e->fork, 0);
+        if (he != hipSuccess) break;
+        rc = c->mixed ? one_step<float>(c) : one_step<double>(c);
+        if (rc) break;
+        he = hipEventRecord(e->join[i], c->stream);
+        if (he == hipSuccess) he = hipStreamWaitEvent(e->origin, e->join[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t he2 = hipStreamEndCapture(e->origin, &g);
+    if (rc || he != hipSuccess || he2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        return fail(LAPS_EDEVICE, std::string("ensemble graph capture: ") +
+                                      hipGetErrorString(he != hipSuccess ? he : he2));
+    }
+    e->graph = g;
+    LAPS_CK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    for (size_t i = 0; i < e->box.size(); ++i) e->gen[i] = e->box[i]->gen;
+    return LAPS_OK;
+}
+
+int laps_ens_create(int nbox, const laps_params* params, laps_ens** out) {
+    if (!out) return fail(LAPS_EINVAL, "null output");
+    *out = nullptr;
+    if (nbox <= 0 || !params) return fail(LAPS_EINVAL, "an ensemble needs nbox >= 1 parameter sets");
+    for (int i = 1; i < nbox; ++i)
+        if (params[i].device != params[0].device) return fail(LAPS_EINVAL, "ensemble boxes must share one device");
+    laps_ens* e = new laps_ens();
+    e->device = params[0].device;
+    auto bail = [&](int rc) {
+        laps_ens_destroy(e);
+        return rc;
+    };
+    for (int i = 0; i < nbox; ++i) {
+        laps_ctx* c = nullptr;
+        int rc = laps_create(&params[i], &c);
+        if (rc) return bail(rc);
+        e->box.push_back(c);
+    }
+    e->gen.assign(nbox, ~0ull);
+    if (hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->after, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(LAPS_EDEVICE, "ensemble stream/event creation failed"));
+    e->join.assign(nbox, nullptr);
+    e->before.assign(nbox, nullptr);
+    for (int i = 0; i < nbox; ++i)
+        if (hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&e->before[i], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(LAPS_EDEVICE, "ensemble event creation failed"));
+    *out = e;
+    return LAPS_OK;
+}
+
+void laps_ens_destroy(laps_ens* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->origin) (void)hipStreamSynchronize(e->origin);
+    ens_drop_graph(e);
+    for (hipEvent_t v : e->join)
+        if (v) (void)hipEventDestroy(v);
+    for (hipEvent_t v : e->before)
+        if (v) (void)hipEventDestroy(v);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->after) (void)hipEventDestroy(e->after);
+    if (e->origin) (void)hipStreamDestroy(e->origin);
+    for (laps_ctx* c : e->box) laps_destroy(c);
+    delete e;
+}
+
+int laps_ens_size(const laps_ens* e) { return e ? (int)e->box.size() : 0; }
+
+laps_ctx* laps_ens_box(laps_ens* e, int i) {
+    if (!e || i < 0 || i >= (int)e->box.size()) return nullptr;
+    return e->box[i];
+}
+
+int laps_ens_step(laps_ens* e, int nsteps) {
+    if (!e) return fail(LAPS_EINVAL, "null ensemble");
+    if (nsteps < 0) return fail(LAPS_EINVAL, "nsteps < 0");
+    if (nsteps == 0) return LAPS_OK;
+    LAPS_CK(hipSetDevice(e->device));
+    for (laps_ctx* c : e->box)
+        if (!c->have_state) return fail(LAPS_ESTATE, "ensemble box without state: call laps_set_state first");
+    // the first step after an upload (and the plan autotune) runs eagerly, for every box alike
+    bool eager = false;
+    for (laps_ctx* c : e->box) eager = eager || c->fresh || !c->tuned;
+    int rc;
+    if (eager) {
+        for (laps_ctx* c : e->box)
+            if ((rc = laps_step(c, 1))) return rc;
+        if (--nsteps == 0) return LAPS_OK;
+    }
+    bool stale = !e->exec;
+    for (size_t i = 0; i < e->box.size(); ++i) stale = stale || e->gen[i] != e->box[i]->gen;
+    // the graph's branches start after all work already queued on the box streams
+    for (size_t i = 0; i < e->box.size(); ++i) {
+        LAPS_CK(hipEventRecord(e->before[i], e->box[i]->stream));
+        LAPS_CK(hipStreamWaitEvent(e->origin, e->before[i], 0));
+    }
+    if (stale) {
+        // capturing records one step without running it; its launches are the graph's
+        std::vector<bool> timing(e->box.size());
+        for (size_t i = 0; i < e->box.size(); ++i) timing[i] = e->box[i]->timing.on;
+        rc = ens_capture(e);
+        for (size_t i = 0; i < e->box.size(); ++i) e->box[i]->timing.on = timing[i];
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) LAPS_CK(hipGraphLaunch(e->exec, e->origin));
+    LAPS_CK(hipEventRecord(e->after, e->origin));
+    for (laps_ctx* c : e->box) {
+        LAPS_CK(hipStreamWaitEvent(c->stream, e->after, 0));
+        c->stepped_since_set = true;
+        c->fresh = false;
+        c->materialized = false;
+    }
+    return LAPS_OK;
+}
+
+int laps_ens_sync(laps_ens* e) {
+    if (!e) return fail(LAPS_EINVAL, "null ensemble");
+    LAPS_CK(hipSetDevice(e->device));
+    LAPS_CK(hipStreamSynchronize(e->origin));
+    for (laps_ctx* c : e->box) LAPS_CK(hipStreamSynchronize(c->stream));
     return LAPS_OK;
 }
 

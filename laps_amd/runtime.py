@@ -23,6 +23,7 @@ EXPORTS = (
     "laps_set_time", "laps_set_fixed_dt", "laps_cfl", "laps_step", "laps_get_scalars", "laps_get_state",
     "laps_get_spectral", "laps_moments", "laps_budgets", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
     "laps_timing_enable", "laps_timing_read", "laps_get_stream",
+    "laps_ens_create", "laps_ens_destroy", "laps_ens_size", "laps_ens_box", "laps_ens_step", "laps_ens_sync",
 )
 
 
@@ -60,6 +61,14 @@ def lib():
         L.laps_timing_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _D, _D,
                                        ctypes.POINTER(ctypes.c_int64)]
         L.laps_get_stream.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p)]
+        L.laps_ens_create.argtypes = [ctypes.c_int, ctypes.POINTER(LapsParams), ctypes.POINTER(vp)]
+        L.laps_ens_destroy.argtypes = [vp]
+        L.laps_ens_destroy.restype = None
+        L.laps_ens_size.argtypes = [vp]
+        L.laps_ens_box.argtypes = [vp, ctypes.c_int]
+        L.laps_ens_box.restype = vp
+        L.laps_ens_step.argtypes = [vp, ctypes.c_int]
+        L.laps_ens_sync.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -77,17 +86,22 @@ def _fptr(a):
 class Laps:
     """One periodic box advanced on the GPU through the C ABI."""
 
-    def __init__(self, params):
+    def __init__(self, params, _handle=None, _owner=None):
         self.params = params
         self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
         self.nxh = self.nx // 2 + 1
+        self._owner = _owner       # an Ensemble owns its boxes' contexts
+        if _handle is not None:
+            self._h = ctypes.c_void_p(_handle)
+            return
         h = ctypes.c_void_p()
         _check(lib().laps_create(ctypes.byref(params), ctypes.byref(h)))
         self._h = h
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().laps_destroy(self._h)
+            if self._owner is None:
+                lib().laps_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -194,3 +208,44 @@ class Laps:
         out = self._real(nf)
         _check(lib().laps_fft_c2r(self._h, nf, _fptr(spec), _fptr(out)))
         return out if spec.ndim == 4 else out[..., 0]
+
+
+class Ensemble:
+    """Independent boxes advanced together (laps_ens_*): box(i) is a Laps view of
+    box i; step(n) advances all of them, one hipGraph launch per step after the
+    first."""
+
+    def __init__(self, params_list):
+        arr = (LapsParams * len(params_list))(*params_list)
+        h = ctypes.c_void_p()
+        _check(lib().laps_ens_create(len(params_list), arr, ctypes.byref(h)))
+        self._h = h
+        self.boxes = [Laps(p, _handle=lib().laps_ens_box(h, i), _owner=self) for i, p in enumerate(params_list)]
+
+    def __len__(self):
+        return len(self.boxes)
+
+    def box(self, i):
+        return self.boxes[i]
+
+    def step(self, n=1):
+        _check(lib().laps_ens_step(self._h, int(n)))
+
+    def sync(self):
+        _check(lib().laps_ens_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for b in self.boxes:
+                b._h = None
+            lib().laps_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
