@@ -51,11 +51,25 @@ __device__ constexpr double kSin16[16] = {
     0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
     -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
 
-// W_R^m = exp(DIR * 2 pi i m / R) for R | 16
+// cos / sin of 2*pi*m/12, m = 0..11
+__device__ constexpr double kCos12[12] = {
+    1.0, 0.86602540378443864676, 0.5, 0.0, -0.5, -0.86602540378443864676,
+    -1.0, -0.86602540378443864676, -0.5, 0.0, 0.5, 0.86602540378443864676};
+__device__ constexpr double kSin12[12] = {
+    0.0, 0.5, 0.86602540378443864676, 1.0, 0.86602540378443864676, 0.5,
+    0.0, -0.5, -0.86602540378443864676, -1.0, -0.86602540378443864676, -0.5};
+
+// W_R^m = exp(DIR * 2 pi i m / R) for R | 16 or R | 12
 template <int R, int DIR, typename T>
 __device__ __forceinline__ cpx<T> wconst(int m) {
-    const int i = (m * (16 / R)) & 15;
-    return {T(kCos16[i]), T(DIR * kSin16[i])};
+    if constexpr (16 % R == 0) {
+        const int i = (m * (16 / R)) & 15;
+        return {T(kCos16[i]), T(DIR * kSin16[i])};
+    } else {
+        static_assert(12 % R == 0, "radix must divide 16 or 12");
+        const int i = (m * (12 / R)) % 12;
+        return {T(kCos12[i]), T(DIR * kSin12[i])};
+    }
 }
 
 template <int R, int DIR, typename T>
@@ -67,6 +81,19 @@ struct Dft<2, DIR, T> {
         cpx<T> a = v[0], b = v[1];
         v[0] = a + b;
         v[1] = a - b;
+    }
+};
+
+// X_1,2 = a - (b + c)/2 -+ DIR i (sqrt(3)/2)(b - c) (DIR = -1: the forward roots)
+template <int DIR, typename T>
+struct Dft<3, DIR, T> {
+    __device__ __forceinline__ static void run(cpx<T>* v) {
+        const cpx<T> s = v[1] + v[2];
+        const cpx<T> h = v[0] - cscale(s, T(0.5));
+        const cpx<T> d = cscale(rotq<DIR>(v[1] - v[2]), T(0.86602540378443864676));
+        v[0] = v[0] + s;
+        v[1] = h + d;
+        v[2] = h - d;
     }
 };
 
@@ -120,6 +147,16 @@ struct Dft<16, DIR, T> {
     __device__ __forceinline__ static void run(cpx<T>* v) { dft_split<4, 4, DIR, T>(v); }
 };
 
+template <int DIR, typename T>
+struct Dft<6, DIR, T> {
+    __device__ __forceinline__ static void run(cpx<T>* v) { dft_split<2, 3, DIR, T>(v); }
+};
+
+template <int DIR, typename T>
+struct Dft<12, DIR, T> {
+    __device__ __forceinline__ static void run(cpx<T>* v) { dft_split<4, 3, DIR, T>(v); }
+};
+
 // Stage plans: radices R0, R1[, R2]. TPL = work-items per line (max butterflies).
 // Plan: two stages where possible (one LDS exchange, 16 values per work-item at 256).
 template <int N>
@@ -130,6 +167,7 @@ template <> struct Plan<64>  { static constexpr int NST = 2, R0 = 8,  R1 = 8,  R
 template <> struct Plan<128> { static constexpr int NST = 2, R0 = 16, R1 = 8,  R2 = 1; };
 template <> struct Plan<256> { static constexpr int NST = 2, R0 = 16, R1 = 16, R2 = 1; };
 template <> struct Plan<512> { static constexpr int NST = 3, R0 = 8,  R1 = 8,  R2 = 8; };
+template <> struct Plan<96>  { static constexpr int NST = 2, R0 = 16, R1 = 6,  R2 = 1; };
 
 // PlanLow: radices <= 8, last stage radix 4 where possible: fewer values per
 // work-item (more work-items per line) for kernels that keep a transform
@@ -142,6 +180,7 @@ template <> struct PlanLow<64>  { static constexpr int NST = 3, R0 = 4, R1 = 4, 
 template <> struct PlanLow<128> { static constexpr int NST = 3, R0 = 8, R1 = 4, R2 = 4; };
 template <> struct PlanLow<256> { static constexpr int NST = 3, R0 = 8, R1 = 8, R2 = 4; };
 template <> struct PlanLow<512> { static constexpr int NST = 3, R0 = 8, R1 = 8, R2 = 8; };
+template <> struct PlanLow<96>  { static constexpr int NST = 3, R0 = 4, R1 = 4, R2 = 6; };
 
 template <int N, class P = Plan<N>>
 struct PlanInfo {

@@ -176,7 +176,7 @@ __global__ __launch_bounds__((XCfg<T, N>::NT)) void k_x_r2c(const T* __restrict_
         const T h = T(0.5);
         for (int k = j; k < a.kx; k += C::TPL) {
             const cpx<T> zk = *lds(k);
-            const cpx<T> zn = *lds((N - k) & (N - 1));
+            const cpx<T> zn = *lds(k == 0 ? 0 : N - k);
             oa[k] = cpx<T>{h * (zk.x + zn.x), h * (zk.y - zn.y)};
             ob[k] = cpx<T>{h * (zk.y + zn.y), h * (zn.x - zk.x)};
         }
@@ -219,6 +219,7 @@ hipError_t launch_strided(int n, int dir, const cpx<TI>* in, cpx<T>* out, const 
         LAPS_STRIDED_CASE(128)
         LAPS_STRIDED_CASE(256)
         LAPS_STRIDED_CASE(512)
+        LAPS_STRIDED_CASE(96)
         default:
             return hipErrorInvalidValue;
     }
@@ -251,6 +252,7 @@ static hipError_t dispatch_x(int n, bool c2r, const void* in, void* out, const X
         case 128: return do_x<T, 128>(c2r, in, out, a, tw, s);
         case 256: return do_x<T, 256>(c2r, in, out, a, tw, s);
         case 512: return do_x<T, 512>(c2r, in, out, a, tw, s);
+        case 96: return do_x<T, 96>(c2r, in, out, a, tw, s);
         default: return hipErrorInvalidValue;
     }
 }

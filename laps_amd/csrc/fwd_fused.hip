@@ -374,6 +374,7 @@ hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs
         case 128: return do_yg<T, 128>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
         case 256: return do_yg<T, 256>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
         case 512: return do_yg<T, 512>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+        case 96: return do_yg<T, 96>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -410,6 +411,7 @@ hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, 
         case 128: return do_zr_opt<T, 128>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
         case 256: return do_zr_opt<T, 256>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
         case 512: return do_zr_opt<T, 512>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
+        case 96: return do_zr_opt<T, 96>(D, U, Fp, a, tw, tb, wzf, pp, ctl, irk, scale, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -90,7 +90,7 @@ double compact_filter(double w, double L, int n, double af) {
     return (aj + bj * cos(wl) + cj * cos(2 * wl)) / (1 + 2 * af * cos(wl));
 }
 
-bool supported_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+bool supported_len(int n) { return n == 16 || n == 32 || n == 64 || n == 96 || n == 128 || n == 256 || n == 512; }
 
 struct DevTables {
     double *wx = nullptr, *wy = nullptr, *wz = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr;
@@ -818,7 +818,7 @@ int laps_create(const laps_params* p, laps_ctx** out) {
     if (!p || !out) return fail(LAPS_EINVAL, "null argument");
     *out = nullptr;
     if (!supported_len(p->nx) || !supported_len(p->ny) || !supported_len(p->nz))
-        return fail(LAPS_EINVAL, "grid lengths must be 16, 32, 64, 128, 256 or 512 per axis");
+        return fail(LAPS_EINVAL, "grid lengths must be 16, 32, 64, 96, 128, 256 or 512 per axis");
     if (p->precision != LAPS_FP64 && p->precision != LAPS_MIXED) return fail(LAPS_EINVAL, "unknown precision");
     if (p->dealias_option < 0 || p->dealias_option > 3)
         return fail(LAPS_EINVAL, "dealias_option must be 0, 1, 2 or 3");

@@ -87,12 +87,63 @@ __device__ __forceinline__ void r4_stages(cpx<T> (&v)[K][4], cpx<T>* const (&lin
 // In-place transform of K rows of data: v[t][r] holds position lane + r*LR
 // on entry and on exit. line[t] is transform t's LDS line. All threads of the
 // block call it.
+// nx = 96 = 4 * 6 * 4: the first and last stages are radix 4 (24 butterflies, one per lane,
+// so that the lane layout of the input and output is that of the other lengths); the middle
+// radix-6 stage (span 4) runs on lanes 0..15 with its operands through LDS.
+template <int DIR, int K, typename T>
+__device__ __forceinline__ void lane_fft_96(cpx<T> (&v)[K][4], cpx<T>* const (&line)[K],
+                                            const cpx<T>* __restrict__ tw, int lane) {
+    using P = LanePad<T>;
+    constexpr int N = 96, LR = 24;
+#pragma unroll
+    for (int t = 0; t < K; ++t) Dft<4, DIR, T>::run(v[t]);
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) *(line[t] + P::at(4 * lane + r)) = v[t][r];
+    __syncthreads();
+    const int k = lane % 4;  // span 4, twiddle step N / (4 * 6) = 4
+    cpx<T> u[K][6];
+    if (lane < 16) {
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) u[t][r] = *(line[t] + P::at(lane + 16 * r));
+#pragma unroll
+            for (int r = 1; r < 6; ++r) u[t][r] = cmul(u[t][r], twiddle<DIR>(tw, 4 * r * k));
+            Dft<6, DIR, T>::run(u[t]);
+        }
+    }
+    __syncthreads();  // every read of the line before any write
+    if (lane < 16) {
+        const int base = (lane / 4) * 24 + k;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+#pragma unroll
+            for (int r = 0; r < 6; ++r) *(line[t] + P::at(base + 4 * r)) = u[t][r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[t][r] = *(line[t] + P::at(lane + r * LR));
+        // span 24: k = lane, twiddle step N / (24 * 4) = 1
+#pragma unroll
+        for (int r = 1; r < 4; ++r) v[t][r] = cmul(v[t][r], twiddle<DIR>(tw, r * lane));
+        Dft<4, DIR, T>::run(v[t]);
+    }
+    static_assert(N == 4 * LR, "lane layout");
+}
+
 template <int N, int DIR, int K, typename T>
 __device__ __forceinline__ void lane_fft(cpx<T> (&v)[K][4], cpx<T>* const (&line)[K], const cpx<T>* __restrict__ tw,
                                          int lane) {
     using P = LanePad<T>;
     constexpr int LR = LaneFft<N>::LR;
-    if constexpr (LaneFft<N>::HAS2) {
+    if constexpr (N == 96) {
+        lane_fft_96<DIR, K, T>(v, line, tw, lane);
+    } else if constexpr (LaneFft<N>::HAS2) {
         // radix-2 stage of span 1: butterfly j = lane pairs r = 0,2; j = lane + LR pairs r = 1,3;
         // outputs of butterfly j go to 2j and 2j + 1
 #pragma unroll
@@ -252,8 +303,11 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     constexpr int LR = LaneFft<N>::LR;
     constexpr int NT = LaneFft<N>::NT;
     constexpr int RPB = NT / LR;  // rows per block
+    // row slots: at nx = 96 a wave holds 2 rows of 24 lanes and 16 lanes more, which run a
+    // dummy row slot of their own (own LDS lines, no stores)
+    constexpr int SLOTS = (NT + LR - 1) / LR;
     constexpr int LS = LanePad<T>::line(N);
-    __shared__ cpx<T> sh[2 * RPB * LS];
+    __shared__ cpx<T> sh[2 * SLOTS * LS];
     __shared__ cpx<T> stw[N];
     const cpx<T>* tw = stw;
     const int lane = threadIdx.x % LR;
@@ -263,9 +317,9 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     const long long gpc = (a.nrows + RPB - 1) / RPB;
     const int ch = (a.nch == 2 && blk >= gpc) ? 1 : 0;
     blk -= ch * gpc;
-    long long row = blk * RPB + rib;
-    const bool valid = row < a.nrows;
-    if (!valid) row = a.nrows - 1;  // computes a duplicate row, never stored
+    long long row = blk * RPB + (rib < RPB ? rib : 0);
+    const bool valid = rib < RPB && row < a.nrows;
+    if (row >= a.nrows) row = a.nrows - 1;  // computes a duplicate row, never stored
     cpx<T>* const line1[1] = {sh + 2 * rib * LS};
     cpx<T>* const line2[2] = {sh + 2 * rib * LS, sh + (2 * rib + 1) * LS};
     // the twiddle table is fetched first and written to LDS once the row's first
@@ -444,6 +498,7 @@ hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cp
         case 128: xf_n<T, 128>(first, cfl, a, tw, pp, ctl, s); break;
         case 256: xf_n<T, 256>(first, cfl, a, tw, pp, ctl, s); break;
         case 512: xf_n<T, 512>(first, cfl, a, tw, pp, ctl, s); break;
+        case 96: xf_n<T, 96>(first, cfl, a, tw, pp, ctl, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -49,7 +49,8 @@ def run_pair(laps_mod, oracle_mod, p, nsteps, seed=0, kmax=None, fixed_dt=None, 
     return g, o
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 64), (64, 32, 16), (128, 16, 16), (16, 16, 256)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 16, 64), (64, 32, 16), (128, 16, 16), (16, 16, 256),
+                                   (96, 32, 16), (16, 96, 96)])
 def test_fft_r2c_c2r_vs_oracle(laps_mod, oracle_mod, shape):
     p = params(shape)
     g = laps_mod.Laps(p)
@@ -133,6 +134,36 @@ def test_phase_shift_noncubic_fp64(laps_mod, oracle_mod):
     err = rel_l2(g.spectral(), o.spectral())
     print(f"phase shift non-cubic rel_l2={err:.3e}")
     assert err < FP64_TOL
+
+
+@pytest.mark.parametrize("case", [
+    dict(n=(96, 32, 64), if_hall=1, ion_inertial_length=0.05),              # radix-3 x pass (dummy lane slot)
+    dict(n=(32, 96, 96), dealias_option=3, if_hall=1, ion_inertial_length=0.05, cfl=0.2),
+    dict(n=(96, 96, 32), dealias_option=2),
+])
+def test_radix3_grids_fp64(laps_mod, oracle_mod, case):
+    case = dict(case)
+    p = params(case.pop("n"), **case)
+    g, o = run_pair(laps_mod, oracle_mod, p, 3, seed=9, kmax=6)
+    err = rel_l2(g.spectral(), o.spectral())
+    print(f"radix-3 case={case} rel_l2={err:.3e}")
+    assert err < FP64_TOL
+    assert g.scalars()["dt"] == pytest.approx(o.scalars()["dt"], rel=1e-13)
+
+
+def test_radix3_mixed(laps_mod, oracle_mod):
+    p = params(96, precision=LAPS_MIXED, if_hall=1, ion_inertial_length=0.05)
+    p64 = params(96, if_hall=1, ion_inertial_length=0.05)
+    uu, pr = seeded_state(p64, seed=10, kmax=8)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p64)
+    g.set_state(uu, pr)
+    o.set_state(uu, pr)
+    g.step(5)
+    o.step(5)
+    err = rel_l2(g.spectral(), o.spectral())
+    print(f"radix-3 mixed 96^3 rel_l2={err:.3e}")
+    assert err < MIXED_TOL
 
 
 def test_config0_64cube_100_steps(laps_mod, oracle_mod):
