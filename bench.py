@@ -24,7 +24,8 @@ products, FP64 RK3 accumulation) is reported under "mixed" with its relative
 L2 distance to the FP64 run after the same number of steps, taken with the
 FP64 run's dt sequence replayed (equal time levels). The same two runs
 with the reference's 2/3 rule (dealias_option 1) are reported under
-"dealias_2_3". `roofline` is the dominant kernel's algorithmic bytes over its
+"dealias_2_3", and the 96-box ensemble (32 each of 32^3, 64^3, 96^3, one
+hipGraph launch per step) under "ensemble". `roofline` is the dominant kernel's algorithmic bytes over its
 HIP-event time; `cpu_baseline` is the CPU oracle timed on this host for three
 steps of the headline workload.
 """
@@ -127,6 +128,95 @@ def dt_replay(n, steps, uu, pr, dealias):
     return ref, spec, dts
 
 
+def ensemble_params(steps_seed=100):
+    """The batched ensemble: 96 independent periodic boxes (32 each of 32^3, 64^3 and 96^3),
+    seeds 100..195, db/B0 = du drawn uniform in [0.1, 0.5], d_i alternating 0 / 0.05, mixed
+    precision (FP32 transforms and nonlinear products, FP64 RK3 accumulation), 2/3 rule, CFL 0.4,
+    nu = eta = 1e-3; k^-5/3 shells 1..n/8."""
+    from laps_amd.params import default_params, LAPS_MIXED
+    rng = np.random.default_rng(steps_seed)
+    out = []
+    for n in (32, 64, 96):
+        for i in range(32):
+            amp = float(rng.uniform(0.1, 0.5))
+            di = 0.05 if i % 2 else 0.0
+            p = default_params(nx=n, ny=n, nz=n, Lx=2 * math.pi, Ly=2 * math.pi, Lz=2 * math.pi,
+                               adiabatic_index=5.0 / 3.0, cfl=0.4, dealias_option=1, if_visc=1, viscosity=1e-3,
+                               if_resis=1, resistivity=1e-3, if_hall=1 if di else 0, ion_inertial_length=di,
+                               precision=LAPS_MIXED)
+            out.append((p, 100 + len(out), amp))
+    return out
+
+
+def run_ensemble(steps):
+    """Box-steps/s of the ensemble advanced by hipGraph replays, and of the same boxes stepped one
+    context at a time (laps_step, each box on its own stream); the two must agree bit for bit."""
+    from laps_amd.ic import cons_from_prim, seeded_primitive
+    from laps_amd.runtime import Ensemble, Laps
+    prev_plan = os.environ.get("LAPS_FFT_PLAN")
+    os.environ["LAPS_FFT_PLAN"] = "0"  # same FFT plan in both runs, so they can be compared bitwise
+    spec = ensemble_params()
+    states = []
+    for p, seed, amp in spec:
+        n = p.nx
+        prim = seeded_primitive((n, n, n), seed, kmin=1, kmax=n // 8, du=amp, db=amp)
+        states.append(cons_from_prim(prim, p.adiabatic_index))
+    log(f"ensemble: {len(spec)} boxes, initial states built")
+    res = {}
+    for graph in (False, True):
+        if graph:
+            ens = Ensemble([p for p, _, _ in spec])
+            boxes = ens.boxes
+        else:
+            boxes = [Laps(p) for p, _, _ in spec]
+        for b, (uu, pr) in zip(boxes, states):
+            b.set_state(uu, pr)
+        # warm-up: the first (per-box) step, the plan set-up and, for the ensemble, the capture
+        if graph:
+            ens.step(2)
+            ens.sync()
+        else:
+            for b in boxes:
+                b.step(2)
+            for b in boxes:
+                b.sync()
+        t0 = time.perf_counter()
+        if graph:
+            ens.step(steps)
+            ens.sync()
+        else:
+            for b in boxes:
+                b.step(steps)
+            for b in boxes:
+                b.sync()
+        dt = time.perf_counter() - t0
+        res[graph] = dict(seconds=dt, scalars=[b.scalars() for b in boxes],
+                          spectra=[boxes[i].spectral() for i in (0, 47, 95)])
+        if graph:
+            ens.close()
+        else:
+            for b in boxes:
+                b.close()
+        log(f"ensemble {'graph' if graph else 'per-box'}: {steps} steps of {len(spec)} boxes in {dt:.3f} s")
+    if prev_plan is None:
+        os.environ.pop("LAPS_FFT_PLAN", None)
+    else:
+        os.environ["LAPS_FFT_PLAN"] = prev_plan
+    g, e = res[True], res[False]
+    same = g["scalars"] == e["scalars"] and all(np.array_equal(a, b) for a, b in zip(g["spectra"], e["spectra"]))
+    pts = sum(p.nx * p.ny * p.nz for p, _, _ in spec)
+    nb = len(spec)
+    return dict(workload="96 independent boxes: 32 x 32^3, 32 x 64^3, 32 x 96^3 (seeds 100-195, db/B0 = du "
+                         "uniform in [0.1, 0.5], d_i 0 / 0.05), mixed precision, 2/3 rule, CFL 0.4, own CFL dt per box",
+                steps=steps, box_steps_per_s=round(nb * steps / g["seconds"], 1),
+                box_steps_per_s_per_box_streams=round(nb * steps / e["seconds"], 1),
+                graph_speedup=round(e["seconds"] / g["seconds"], 3),
+                grid_point_steps_per_s=round(pts * steps / g["seconds"], 1),
+                ms_per_ensemble_step=round(1000.0 * g["seconds"] / steps, 4),
+                graph_equals_per_box_steps=bool(same),
+                time_range=[min(x["time"] for x in g["scalars"]), max(x["time"] for x in g["scalars"])])
+
+
 def roofline(timing, precision):
     name, (ms, by, nl) = max(timing.items(), key=lambda kv: kv[1][0])
     per_launch_s = ms * 1e-3 / max(nl, 1)
@@ -174,6 +264,8 @@ def main():
                     help="3: phase-shift dealiasing (the workload's own); 1: the reference's 2/3 rule")
     ap.add_argument("--no-compare", action="store_true", help="skip the runs with the other dealiasing option")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="steps of the 96-box ensemble reported under 'ensemble' (0: skip it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run y = 2x over 1 GiB (1 GiB read + 1 GiB written) for HBM counter calibration")
@@ -276,6 +368,8 @@ def main():
         for v in ro.values():
             v.pop("spectral", None)
         out["dealias_2_3" if other == 1 else "dealias_phase_shift"] = rep
+    if args.ensemble_steps > 0:
+        out["ensemble"] = run_ensemble(args.ensemble_steps)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
