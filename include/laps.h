@@ -196,6 +196,37 @@ laps_ctx* laps_ens_box(laps_ens* ens, int i);
 int laps_ens_step(laps_ens* ens, int nsteps);
 int laps_ens_sync(laps_ens* ens);
 
+/* Learned subgrid closure. No counterpart in the reference, whose fluxes are
+ * the resolved ones only (mhdrhs.f90:60-119); callers: coarse-grid runs
+ * that stand in for finer ones. Features per point (28): rho, u, B and
+ * their central-difference gradients d/dx, d/dy, d/dz (rho, ux, uy, uz,
+ * Bx, By, Bz), each (f - mean)/std. Network 28 -> 64 (tanh) -> 64 (tanh) ->
+ * 15 in FP32: subgrid Reynolds stress (xx, xy, xz, yy, yz, zz), Maxwell
+ * stress (same order) and EMF (x, y, z), each times ystd. Parameter vector:
+ * W1[28][64], b1[64], W2[64][64], b2[64], W3[64][15], b3[15] (the network,
+ * LAPS_MLP_NPARAM floats), then xmean[28], xstd[28], ystd[15].
+ * laps_closure_set (NULL: off) makes every following step evaluate the
+ * network on the state at the start of the step and add -div(R - M + q I)
+ * to the momentum and -curl(EMF) to the induction right-hand side of its
+ * three substages, with q the subgrid part of p + B^2/2. AEB is not
+ * supported. laps_closure_features returns the raw (unnormalised) features
+ * (nx*ny*nz, 28); laps_closure_eval the normalised outputs (nx*ny*nz, 15)
+ * and, if rhs != NULL, the RHS term (nx/2+1, ny, nz, 6) of the current state.
+ * laps_mlp_*: the network on caller samples (rows of 28 inputs, 15 targets)
+ * on device `device`: forward; gradient of the mean squared error with its
+ * loss; `steps` Adam steps (lr, betas 0.9/0.999) over consecutive batches of
+ * the (caller-shuffled) samples, with the loss of each batch. */
+#define LAPS_MLP_NPARAM 6991
+#define LAPS_CLOSURE_NPARAM (LAPS_MLP_NPARAM + 71)
+int laps_closure_set(laps_ctx* ctx, const float* params, int nparams);
+int laps_closure_features(laps_ctx* ctx, float* features);
+int laps_closure_eval(laps_ctx* ctx, float* outputs, double* rhs);
+int laps_mlp_forward(int device, int64_t n, const float* X, const float* params, float* Y);
+int laps_mlp_grad(int device, int64_t n, const float* X, const float* T, const float* params, float* grad,
+                  float* loss);
+int laps_mlp_train(int device, int64_t n, const float* X, const float* T, float* params, int steps, int batch,
+                   float lr, float* loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

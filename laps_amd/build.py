@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "liblaps_amd.so")
 ARCH = os.environ.get("LAPS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fft_kernels.hip", "xfused.hip", "fwd_fused.hip", "physics_kernels.hip", "laps_api.hip"]
+SOURCES = ["fft_kernels.hip", "xfused.hip", "fwd_fused.hip", "physics_kernels.hip", "closure.hip", "laps_api.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

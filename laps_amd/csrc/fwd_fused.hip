@@ -311,6 +311,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
             const double wz = swz[p];
             const cpx<T> fv = *lds(p);
             cpx<double> fnl{double(fv.x) * double(scale), double(fv.y) * double(scale)};
+            if (a.clos && v >= 1 && v <= 6) fnl = fnl + a.clos[(v - 1) * a.ufs + mi];
             const cpx<double> u = U[v * a.ufs + mi];
             if (pp.aeb && v < 7) {
                 const double fac = (v == 2 || v == 3) ? 3.0 : (v >= 5 ? 1.0 : 2.0);

@@ -156,6 +156,7 @@ struct ZArgs {
     const cpx<double>* ph;    // fnl = (Z(D) + conj(ph[p] phc[kx] phb[cy]) Z(Ds)) / 2 at output mode
     const cpx<double>* phc;   // (kx, cy, p)
     const cpx<double>* phb;
+    const cpx<double>* clos;  // optional learned-closure RHS [6][modes] added to momentum and B (v = 1..6)
 };
 template <typename T>
 hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,
@@ -190,5 +191,21 @@ hipError_t launch_gather_box(const cpx<double>* full, cpx<double>* box, int nf, 
                              long long box_stride, int nxh, int ny, int nz, Box b, hipStream_t s);
 template <typename TI, typename TO>
 hipError_t launch_convert(const TI* in, TO* out, long long n, hipStream_t s);
+
+// --- learned closure (closure.hip) ---
+constexpr int kClIn = 28, kClH = 64, kClOut = 15;
+constexpr int kClNParam = kClIn * kClH + kClH + kClH * kClH + kClH + kClH * kClOut + kClOut;
+constexpr int kClNNorm = 2 * kClIn + kClOut;  // xmean, xstd, ystd
+hipError_t launch_cl_features(const double* S, int nx, int ny, int nz, double dx, double dy, double dz,
+                              const float* norm, float* X, hipStream_t s);
+hipError_t launch_mlp_fwd(const float* X, long long n, const float* P, float* Y, float* H1, float* H2,
+                          hipStream_t s);
+hipError_t launch_mlp_grad(const float* X, const float* T, long long n, const float* P, const float* Y,
+                           const float* H1, const float* H2, float* scratch, float* G, float* loss, hipStream_t s);
+hipError_t launch_adam(float* P, const float* G, float* m, float* v, int n, float lr, float b1t, float b2t,
+                       hipStream_t s);
+hipError_t launch_cl_terms(const float* Y, long long N, const float* ystd, double gamma, double* C, hipStream_t s);
+hipError_t launch_cl_rhs(const cpx<double>* Ch, long long M, int nxh, int ny, const double* wx, const double* wy,
+                         const double* wz, cpx<double>* R, hipStream_t s);
 
 }  // namespace laps

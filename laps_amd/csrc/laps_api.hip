@@ -31,6 +31,8 @@
 using namespace laps;
 
 static_assert(sizeof(laps_params) == 168, "laps_params layout is part of the ABI");
+static_assert(LAPS_MLP_NPARAM == laps::kClNParam && LAPS_CLOSURE_NPARAM == laps::kClNParam + laps::kClNNorm,
+              "closure parameter layout");
 
 namespace {
 
@@ -137,6 +139,12 @@ struct laps_ctx {
     void* P2 = nullptr;
     void* P3 = nullptr;                                   // shifted-grid y-pass output (option 3), in P2
     long long p1 = 0, p2 = 0;                             // complex elements per chain of P1, P2
+    // learned closure: parameters + normalisation (FP32), features, outputs, the 9 real
+    // correction fields, their spectra, and the RHS term on the full and the kept box
+    bool closure = false;
+    float *cl_P = nullptr, *cl_X = nullptr, *cl_Y = nullptr, *cl_id = nullptr;
+    double* cl_C = nullptr;
+    cpx<double>*cl_hat = nullptr, *cl_full = nullptr, *cl_rhs = nullptr;
     cpx<double>*phx = nullptr, *phy = nullptr, *phz = nullptr;  // e^{i pi m/n} per axis (option 3)
     cpx<double>* phyk = nullptr;                                 // phy of the kept box's stored ky
     cpx<double>* X1 = nullptr;
@@ -519,6 +527,7 @@ int substage(laps_ctx* c, int irk) {
                                         c->nz, c->stream);
         }));
     }
+    za.clos = c->closure ? c->cl_rhs : nullptr;
     if (c->p.dealias_option == 3) {
         // the z pass averages the right-hand sides of the two grids
         za.Ds = P3;
@@ -593,9 +602,12 @@ int autotune(laps_ctx* c) {
     return LAPS_OK;
 }
 
+int closure_step(laps_ctx* c);
+
 template <typename T>
 int one_step(laps_ctx* c) {
     int rc;
+    if (c->closure && (rc = closure_step(c))) return rc;
     for (int irk = 0; irk < 3; ++irk)
         if ((rc = substage<T>(c, irk))) return rc;
     LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_end(c->ctl, c->pp, c->stream); }));
@@ -643,6 +655,45 @@ int materialize(laps_ctx* c) {
         if ((rc = inv_fp64(c, c->U + f * c->kb(), c->kept, c->S + f * c->N))) return rc;
     LAPS_CK(launch_prim<double>(c->S, c->S + 8 * c->N, c->N, c->p.adiabatic_index, c->stream));
     c->materialized = true;
+    return LAPS_OK;
+}
+
+// Learned closure, evaluated once per step on the state at the start of the
+// step: features -> network -> 9 correction fields -> their spectra -> the
+// momentum and induction RHS term on the kept box (held over the three
+// substages; the z pass adds it).
+int closure_buffers(laps_ctx* c) {
+    if (c->cl_P) return LAPS_OK;
+    int rc;
+    if ((rc = c->allocn(&c->cl_P, kClNParam + kClNNorm)) || (rc = c->allocn(&c->cl_X, c->N * kClIn)) ||
+        (rc = c->allocn(&c->cl_Y, c->N * kClOut)) || (rc = c->allocn(&c->cl_id, 2 * kClIn)) ||
+        (rc = c->allocn(&c->cl_C, 9 * c->N)) || (rc = c->allocn(&c->cl_hat, 9 * c->M)) ||
+        (rc = c->allocn(&c->cl_full, 6 * c->M)) || (rc = c->allocn(&c->cl_rhs, 6 * c->kb())))
+        return rc;
+    std::vector<float> id(2 * kClIn, 0.f);
+    for (int q = 0; q < kClIn; ++q) id[kClIn + q] = 1.f;
+    LAPS_CK(hipMemcpy(c->cl_id, id.data(), id.size() * sizeof(float), hipMemcpyHostToDevice));
+    return LAPS_OK;
+}
+
+int closure_features(laps_ctx* c, const float* norm) {
+    int rc = materialize(c);
+    if (rc) return rc;
+    LAPS_CK(launch_cl_features(c->S, c->nx, c->ny, c->nz, c->pp.dx, c->pp.dy, c->pp.dz, norm, c->cl_X, c->stream));
+    return LAPS_OK;
+}
+
+int closure_step(laps_ctx* c) {
+    int rc = closure_features(c, c->cl_P + kClNParam);
+    if (rc) return rc;
+    LAPS_CK(launch_mlp_fwd(c->cl_X, c->N, c->cl_P, c->cl_Y, nullptr, nullptr, c->stream));
+    LAPS_CK(launch_cl_terms(c->cl_Y, c->N, c->cl_P + kClNParam + 2 * kClIn, c->p.adiabatic_index, c->cl_C,
+                            c->stream));
+    for (int f = 0; f < 9; ++f)
+        if ((rc = fwd_fp64(c, c->cl_C + f * c->N, c->cl_hat + f * c->M))) return rc;
+    LAPS_CK(launch_cl_rhs(c->cl_hat, c->M, c->nxh, c->ny, c->tfull.wx, c->tfull.wy, c->tfull.wz, c->cl_full,
+                          c->stream));
+    LAPS_CK(launch_gather_box(c->cl_full, c->cl_rhs, 6, c->M, c->kb(), c->nxh, c->ny, c->nz, c->kept, c->stream));
     return LAPS_OK;
 }
 
@@ -1261,6 +1312,8 @@ static void ens_drop_graph(laps_ens* e) {
 
 static int ens_capture(laps_ens* e) {
     ens_drop_graph(e);
+    for (laps_ctx* c : e->box)
+        if (c->closure) return fail(LAPS_EINVAL, "ensemble boxes with a learned closure are stepped per box");
     for (laps_ctx* c : e->box) c->timing.on = false;
     LAPS_CK(hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
     int rc = LAPS_OK;
@@ -1391,6 +1444,145 @@ int laps_ens_sync(laps_ens* e) {
     LAPS_CK(hipSetDevice(e->device));
     LAPS_CK(hipStreamSynchronize(e->origin));
     for (laps_ctx* c : e->box) LAPS_CK(hipStreamSynchronize(c->stream));
+    return LAPS_OK;
+}
+
+// ------------------------------------------------------------------ learned closure
+
+int laps_closure_set(laps_ctx* c, const float* params, int nparams) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (!params) {
+        c->closure = false;
+        ++c->gen;
+        return LAPS_OK;
+    }
+    if (nparams != LAPS_CLOSURE_NPARAM) return fail(LAPS_EINVAL, "closure parameter vector has the wrong length");
+    if (c->p.if_aeb) return fail(LAPS_EINVAL, "the learned closure is defined for the periodic box without AEB");
+    if ((rc = closure_buffers(c))) return rc;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(c->cl_P, params, (size_t)nparams * sizeof(float), hipMemcpyHostToDevice));
+    c->closure = true;
+    ++c->gen;
+    return LAPS_OK;
+}
+
+int laps_closure_features(laps_ctx* c, float* out) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!out) return fail(LAPS_EINVAL, "null output");
+    if ((rc = closure_buffers(c))) return rc;
+    if ((rc = closure_features(c, c->cl_id))) return rc;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    LAPS_CK(hipMemcpy(out, c->cl_X, (size_t)c->N * kClIn * sizeof(float), hipMemcpyDeviceToHost));
+    return LAPS_OK;
+}
+
+int laps_closure_eval(laps_ctx* c, float* out, double* rhs) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!c->closure) return fail(LAPS_ESTATE, "no closure set: call laps_closure_set first");
+    if ((rc = closure_step(c))) return rc;
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    if (out) LAPS_CK(hipMemcpy(out, c->cl_Y, (size_t)c->N * kClOut * sizeof(float), hipMemcpyDeviceToHost));
+    if (rhs) LAPS_CK(hipMemcpy(rhs, c->cl_full, (size_t)6 * c->M * sizeof(cpx<double>), hipMemcpyDeviceToHost));
+    return LAPS_OK;
+}
+
+// Device buffers of the stateless network calls, freed on every path.
+struct MlpBufs {
+    std::vector<void*> ptrs;
+    int get(float** p, size_t count) {
+        void* q = nullptr;
+        if (hipMalloc(&q, count * sizeof(float) + 16) != hipSuccess)
+            return fail(LAPS_ENOMEM, "network buffer allocation failed");
+        ptrs.push_back(q);
+        *p = (float*)q;
+        return LAPS_OK;
+    }
+    ~MlpBufs() {
+        for (void* q : ptrs) (void)hipFree(q);
+    }
+};
+
+int laps_mlp_forward(int device, int64_t n, const float* X, const float* params, float* Y) {
+    if (n <= 0 || !X || !params || !Y) return fail(LAPS_EINVAL, "bad network arguments");
+    LAPS_CK(hipSetDevice(device));
+    MlpBufs b;
+    float *dX, *dP, *dY;
+    int rc;
+    if ((rc = b.get(&dX, n * kClIn)) || (rc = b.get(&dP, kClNParam)) || (rc = b.get(&dY, n * kClOut))) return rc;
+    LAPS_CK(hipMemcpy(dX, X, n * kClIn * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(dP, params, kClNParam * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(launch_mlp_fwd(dX, n, dP, dY, nullptr, nullptr, nullptr));
+    LAPS_CK(hipDeviceSynchronize());
+    LAPS_CK(hipMemcpy(Y, dY, n * kClOut * sizeof(float), hipMemcpyDeviceToHost));
+    return LAPS_OK;
+}
+
+int laps_mlp_grad(int device, int64_t n, const float* X, const float* T, const float* params, float* grad,
+                  float* loss) {
+    if (n <= 0 || !X || !T || !params || !grad) return fail(LAPS_EINVAL, "bad network arguments");
+    LAPS_CK(hipSetDevice(device));
+    MlpBufs b;
+    float *dX, *dT, *dP, *dY, *dH1, *dH2, *dS, *dG, *dL;
+    int rc;
+    if ((rc = b.get(&dX, n * kClIn)) || (rc = b.get(&dT, n * kClOut)) || (rc = b.get(&dP, kClNParam)) ||
+        (rc = b.get(&dY, n * kClOut)) || (rc = b.get(&dH1, n * kClH)) || (rc = b.get(&dH2, n * kClH)) ||
+        (rc = b.get(&dS, n * (2 * kClH + kClOut + 1))) || (rc = b.get(&dG, kClNParam)) || (rc = b.get(&dL, 1)))
+        return rc;
+    LAPS_CK(hipMemcpy(dX, X, n * kClIn * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(dT, T, n * kClOut * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(dP, params, kClNParam * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(launch_mlp_fwd(dX, n, dP, dY, dH1, dH2, nullptr));
+    LAPS_CK(launch_mlp_grad(dX, dT, n, dP, dY, dH1, dH2, dS, dG, dL, nullptr));
+    LAPS_CK(hipDeviceSynchronize());
+    LAPS_CK(hipMemcpy(grad, dG, kClNParam * sizeof(float), hipMemcpyDeviceToHost));
+    float l = 0.f;
+    LAPS_CK(hipMemcpy(&l, dL, sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) *loss = l / (float)((double)n * kClOut);
+    return LAPS_OK;
+}
+
+int laps_mlp_train(int device, int64_t n, const float* X, const float* T, float* params, int steps, int batch,
+                   float lr, float* loss_hist) {
+    if (n <= 0 || !X || !T || !params || steps < 0 || batch <= 0) return fail(LAPS_EINVAL, "bad training arguments");
+    if (batch > n) batch = (int)n;
+    LAPS_CK(hipSetDevice(device));
+    MlpBufs b;
+    float *dX, *dT, *dP, *dY, *dH1, *dH2, *dS, *dG, *dM, *dV, *dL;
+    int rc;
+    if ((rc = b.get(&dX, n * kClIn)) || (rc = b.get(&dT, n * kClOut)) || (rc = b.get(&dP, kClNParam)) ||
+        (rc = b.get(&dY, (size_t)batch * kClOut)) || (rc = b.get(&dH1, (size_t)batch * kClH)) ||
+        (rc = b.get(&dH2, (size_t)batch * kClH)) || (rc = b.get(&dS, (size_t)batch * (2 * kClH + kClOut + 1))) ||
+        (rc = b.get(&dG, kClNParam)) || (rc = b.get(&dM, kClNParam)) || (rc = b.get(&dV, kClNParam)) ||
+        (rc = b.get(&dL, steps > 0 ? steps : 1)))
+        return rc;
+    LAPS_CK(hipMemcpy(dX, X, n * kClIn * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(dT, T, n * kClOut * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(dP, params, kClNParam * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> zero(kClNParam, 0.f);
+    LAPS_CK(hipMemcpy(dM, zero.data(), kClNParam * sizeof(float), hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(dV, zero.data(), kClNParam * sizeof(float), hipMemcpyHostToDevice));
+    const long long nb = n / batch;  // whole batches, taken in turn (the caller shuffles the samples)
+    double b1t = 1.0, b2t = 1.0;
+    for (int s = 0; s < steps; ++s) {
+        const long long off = (long long)(s % nb) * batch;
+        const float* xb = dX + off * kClIn;
+        const float* tb = dT + off * kClOut;
+        LAPS_CK(launch_mlp_fwd(xb, batch, dP, dY, dH1, dH2, nullptr));
+        LAPS_CK(launch_mlp_grad(xb, tb, batch, dP, dY, dH1, dH2, dS, dG, dL + s, nullptr));
+        b1t *= 0.9;
+        b2t *= 0.999;
+        LAPS_CK(launch_adam(dP, dG, dM, dV, kClNParam, lr, (float)b1t, (float)b2t, nullptr));
+    }
+    LAPS_CK(hipDeviceSynchronize());
+    LAPS_CK(hipMemcpy(params, dP, kClNParam * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss_hist && steps > 0) {
+        std::vector<float> h(steps);
+        LAPS_CK(hipMemcpy(h.data(), dL, steps * sizeof(float), hipMemcpyDeviceToHost));
+        for (int s = 0; s < steps; ++s) loss_hist[s] = h[s] / (float)((double)batch * kClOut);
+    }
     return LAPS_OK;
 }
 

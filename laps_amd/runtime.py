@@ -24,7 +24,13 @@ EXPORTS = (
     "laps_get_spectral", "laps_moments", "laps_budgets", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
     "laps_timing_enable", "laps_timing_read", "laps_get_stream",
     "laps_ens_create", "laps_ens_destroy", "laps_ens_size", "laps_ens_box", "laps_ens_step", "laps_ens_sync",
+    "laps_closure_set", "laps_closure_features", "laps_closure_eval", "laps_mlp_forward", "laps_mlp_grad",
+    "laps_mlp_train",
 )
+
+MLP_NPARAM = 6991          # LAPS_MLP_NPARAM
+CLOSURE_NPARAM = 6991 + 71  # LAPS_CLOSURE_NPARAM
+_F = ctypes.POINTER(ctypes.c_float)
 
 
 class LapsError(RuntimeError):
@@ -69,6 +75,13 @@ def lib():
         L.laps_ens_box.restype = vp
         L.laps_ens_step.argtypes = [vp, ctypes.c_int]
         L.laps_ens_sync.argtypes = [vp]
+        L.laps_closure_set.argtypes = [vp, _F, ctypes.c_int]
+        L.laps_closure_features.argtypes = [vp, _F]
+        L.laps_closure_eval.argtypes = [vp, _F, _D]
+        L.laps_mlp_forward.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F]
+        L.laps_mlp_grad.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, _F, _F]
+        L.laps_mlp_train.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_float, _F]
         _lib = L
     return _lib
 
@@ -81,6 +94,47 @@ def _check(rc):
 
 def _fptr(a):
     return a.ctypes.data_as(_D)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        assert a.shape == shape, (a.shape, shape)
+    return a
+
+
+def _sptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_forward(X, params, device=0):
+    """Closure network on samples X (n, 28) -> (n, 15)."""
+    X = _f32(X)
+    P = _f32(params, (MLP_NPARAM,))
+    Y = np.zeros((X.shape[0], 15), np.float32)
+    _check(lib().laps_mlp_forward(device, X.shape[0], _sptr(X), _sptr(P), _sptr(Y)))
+    return Y
+
+
+def mlp_grad(X, T, params, device=0):
+    """Gradient of the mean squared error over all samples, and the loss."""
+    X, T = _f32(X), _f32(T)
+    P = _f32(params, (MLP_NPARAM,))
+    G = np.zeros(MLP_NPARAM, np.float32)
+    loss = ctypes.c_float()
+    _check(lib().laps_mlp_grad(device, X.shape[0], _sptr(X), _sptr(T), _sptr(P), _sptr(G), ctypes.byref(loss)))
+    return G, float(loss.value)
+
+
+def mlp_train(X, T, params, steps, batch, lr, device=0):
+    """Adam steps over consecutive batches of the (already shuffled) samples; returns the new
+    parameters and the loss of every batch."""
+    X, T = _f32(X), _f32(T)
+    P = _f32(params, (MLP_NPARAM,)).copy()
+    hist = np.zeros(max(steps, 1), np.float32)
+    _check(lib().laps_mlp_train(device, X.shape[0], _sptr(X), _sptr(T), _sptr(P), int(steps), int(batch),
+                                float(lr), _sptr(hist)))
+    return P, hist[:steps]
 
 
 class Laps:
@@ -194,6 +248,27 @@ class Laps:
                 return out
             out[name.value.decode()] = (ms.value, by.value, n.value)
             i += 1
+
+    def closure_set(self, params):
+        """Learned closure on (params: CLOSURE_NPARAM floats) or off (None)."""
+        if params is None:
+            _check(lib().laps_closure_set(self._h, None, 0))
+            return
+        P = _f32(params, (CLOSURE_NPARAM,))
+        _check(lib().laps_closure_set(self._h, _sptr(P), P.size))
+
+    def closure_features(self):
+        """Raw features (nx*ny*nz, 28) of the current state, points in Fortran order."""
+        out = np.zeros((self.nx * self.ny * self.nz, 28), np.float32)
+        _check(lib().laps_closure_features(self._h, _sptr(out)))
+        return out
+
+    def closure_eval(self):
+        """Normalised network outputs (nx*ny*nz, 15) and the closure RHS term (nxh, ny, nz, 6)."""
+        out = np.zeros((self.nx * self.ny * self.nz, 15), np.float32)
+        rhs = self._spec(6)
+        _check(lib().laps_closure_eval(self._h, _sptr(out), _fptr(rhs)))
+        return out, rhs
 
     def fft_r2c(self, real):
         real = np.asfortranarray(real, dtype=np.float64)
