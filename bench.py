@@ -25,7 +25,10 @@ L2 distance to the FP64 run after the same number of steps, taken with the
 FP64 run's dt sequence replayed (equal time levels). The same two runs
 with the reference's 2/3 rule (dealias_option 1) are reported under
 "dealias_2_3", and the 96-box ensemble (32 each of 32^3, 64^3, 96^3, one
-hipGraph launch per step) under "ensemble". `roofline` is the dominant kernel's algorithmic bytes over its
+hipGraph launch per step) under "ensemble". "closure" is the learned subgrid
+closure trained on box-filtered 128^3 runs and run on 32^3: shell-energy
+deviations from the filtered fine run with and without it, and its cost
+against the fine steps it replaces. `roofline` is the dominant kernel's algorithmic bytes over its
 HIP-event time; `cpu_baseline` is the CPU oracle timed on this host for three
 steps of the headline workload.
 """
@@ -266,6 +269,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps of the 96-box ensemble reported under 'ensemble' (0: skip it)")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the learned-closure experiment reported under 'closure'")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run y = 2x over 1 GiB (1 GiB read + 1 GiB written) for HBM counter calibration")
@@ -370,6 +375,9 @@ def main():
         out["dealias_2_3" if other == 1 else "dealias_phase_shift"] = rep
     if args.ensemble_steps > 0:
         out["ensemble"] = run_ensemble(args.ensemble_steps)
+    if not args.no_closure:
+        from laps_amd.closure import experiment
+        out["closure"] = experiment(log=log)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:

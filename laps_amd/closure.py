@@ -144,3 +144,114 @@ def shell_spectrum(uhat, n, kmax):
     e = sum(np.abs(uhat[..., v]) ** 2 for v in (1, 2, 3, 4, 5, 6)) * 0.5 * w
     shell = np.rint(kk).astype(np.int64).ravel()
     return np.bincount(shell, weights=e.ravel(), minlength=kmax + 1)[: kmax + 1]
+
+
+def fine_run(p_fine, seed, r, nsteps, every, amp, dt):
+    """Snapshots (step, uu, prim) of a fine run at a fixed dt, every `every` steps."""
+    from .ic import cons_from_prim, seeded_primitive
+    n = (p_fine.nx, p_fine.ny, p_fine.nz)
+    uu, pr = cons_from_prim(seeded_primitive(n, seed, kmin=1, kmax=8, du=amp, db=amp), p_fine.adiabatic_index)
+    out = []
+    with Laps(p_fine) as g:
+        g.set_state(uu, pr)
+        g.set_fixed_dt(dt)
+        for s in range(0, nsteps + 1, every):
+            if s:
+                g.step(every)
+            u, q = g.state()
+            out.append((s, u, q))
+    return out
+
+
+def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, every=32, amp=0.4,
+               di=0.05, visc=5e-3, train_steps=4000, log=print):
+    """Trains the closure on box-filtered fine runs and compares coarse runs with and
+    without it against the filtered fine run of another seed: energy spectra per shell up
+    to the coarse grid's dealiased range, and the cost of a coarse step with the closure
+    against the fine steps it stands in for."""
+    import time
+    from .params import default_params
+    L = 2.0 * math.pi
+    pf = default_params(nx=n_fine, ny=n_fine, nz=n_fine, Lx=L, Ly=L, Lz=L, adiabatic_index=5.0 / 3.0, cfl=0.4,
+                        dealias_option=1, if_visc=1, viscosity=visc, if_resis=1, resistivity=visc,
+                        if_hall=1 if di else 0, ion_inertial_length=di)
+    pc = coarse_params(pf, r)
+    nc = pc.nx
+    # the fine dt: 0.8 of the CFL dt of the first training state, kept for every run
+    from .ic import cons_from_prim, seeded_primitive
+    uu0, pr0 = cons_from_prim(seeded_primitive((n_fine,) * 3, train_seeds[0], kmin=1, kmax=8, du=amp, db=amp),
+                              pf.adiabatic_index)
+    with Laps(pf) as g:
+        g.set_state(uu0, pr0)
+        dt = 0.8 * g.cfl()[1]
+    t0 = time.perf_counter()
+    X, T = [], []
+    for sd in train_seeds:
+        for s, u, q in fine_run(pf, sd, r, nsteps, every, amp, dt):
+            uc, qc, t = filtered_sample(u, q, r, pf.adiabatic_index, di, (L, L, L))
+            X.append(features_of(pc, uc, qc))
+            T.append(t)
+    X = np.concatenate(X)
+    T = np.concatenate(T).astype(np.float32)
+    log(f"closure: {X.shape[0]} training samples from {len(train_seeds)} fine {n_fine}^3 runs "
+        f"({time.perf_counter() - t0:.1f} s)")
+    t1 = time.perf_counter()
+    vec, hist = train(X, T, steps=train_steps)
+    log(f"closure: trained {train_steps} Adam steps in {time.perf_counter() - t1:.1f} s, "
+        f"loss {hist[:50].mean():.3f} -> {hist[-50:].mean():.3f}")
+    # evaluation on an unseen seed
+    snaps = fine_run(pf, test_seed, r, nsteps, nsteps, amp, dt)
+    (_, u0, q0), (_, u1, q1) = snaps[0], snaps[-1]
+    uc0, qc0, _ = filtered_sample(u0, q0, r, pf.adiabatic_index, di, (L, L, L))
+    uc1, _, T1 = filtered_sample(u1, q1, r, pf.adiabatic_index, di, (L, L, L))
+    kmax = nc // 3
+    with Laps(pc) as ref_g:
+        ref_g.set_state(uc1, filtered_sample(u1, q1, r, pf.adiabatic_index, di, (L, L, L))[1])
+        e_ref = shell_spectrum(ref_g.spectral(), nc, kmax)
+    res = {}
+    for tag, use in (("plain", False), ("closure", True)):
+        with Laps(pc) as g:
+            g.set_state(uc0, qc0)
+            g.set_fixed_dt(r * dt)
+            if use:
+                g.closure_set(vec)
+            g.step(nsteps // r)
+            g.sync()
+            t2 = time.perf_counter()
+            g.step(nsteps // r)
+            g.sync()
+            tc = (time.perf_counter() - t2) / (nsteps // r)
+            g.closure_set(None)
+            g.set_state(uc0, qc0)
+            g.set_fixed_dt(r * dt)
+            if use:
+                g.closure_set(vec)
+            g.step(nsteps // r)
+            e = shell_spectrum(g.spectral(), nc, kmax)
+        dev = np.abs(e[1:] - e_ref[1:]) / e_ref[1:]
+        res[tag] = dict(max_shell_dev=float(dev.max()), mean_shell_dev=float(dev.mean()),
+                        shell_dev=[round(float(x), 4) for x in dev], s_per_step=tc)
+        log(f"closure: coarse {nc}^3 {tag}: max shell deviation {dev.max():.4f}, mean {dev.mean():.4f}")
+    with Laps(pf) as g:
+        g.set_state(u0, q0)
+        g.set_fixed_dt(dt)
+        g.step(4)
+        g.sync()
+        t3 = time.perf_counter()
+        g.step(nsteps // 4)
+        g.sync()
+        tf = (time.perf_counter() - t3) / (nsteps // 4)
+    cost = (tf * r) / res["closure"]["s_per_step"]
+    return dict(workload=f"fine {n_fine}^3 -> coarse {nc}^3 (box filter of {r}^3 cells), Hall d_i={di}, "
+                         f"nu=eta={visc}, seeded k^-5/3 shells 1-8 rms {amp}; trained on seeds {list(train_seeds)}, "
+                         f"evaluated on seed {test_seed} after {nsteps} fine steps",
+                samples=int(X.shape[0]), train_steps=train_steps,
+                loss_first=float(hist[:50].mean()), loss_last=float(hist[-50:].mean()),
+                shells=f"1..{kmax}", plain=res["plain"], closure=res["closure"],
+                cost_ratio_fine_over_coarse_closure=round(float(cost), 2))
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    print(json.dumps(experiment(log=lambda m: print(m, file=sys.stderr, flush=True))))
