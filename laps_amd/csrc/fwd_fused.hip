@@ -15,15 +15,19 @@
 // curls P5 = -i ky' Y(E_z), P6 = i kx' Y(E_z), P7 = i ky' Y(E_x) - i kx' Y(E_y).
 // With corotating AEB, kx' and ky' both depend on (kx, ky); both are known per
 // output mode of the y pass.
+#include <type_traits>
+
 #include "internal.hpp"
 #include "physics_dev.hpp"
 
 namespace laps {
 
-// ZP: the z pass, whose blocks also carry the RHS and RK3 update
+// ZP: the z pass, whose blocks also carry the RHS and RK3 update. The FP32 y pass at 256 uses
+// the two-stage radix-16x16 plan (one LDS exchange; FP32 has the registers for 16 values per
+// work-item): 3.09 -> 2.74 ms per phase-shift step. The z pass keeps PlanLow (slower with it).
 template <typename T, int N, bool ZP>
 struct FwdCfg {
-    using P = PlanLow<N>;
+    using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP), Plan<N>, PlanLow<N>>;
     using I = PlanInfo<N, P>;
     // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns) in the y pass; the z
     // pass keeps 8 FP32 columns, whose blocks of 8 waves leave room for more per CU
@@ -347,6 +351,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
                 un = {((un.x * fx) * fy) * fz, ((un.y * fx) * fy) * fz};
             }
             U[v * a.ufs + mi] = un;
+            if (a.Uf) a.Uf[v * a.ufs + mi] = cpx<float>{float(un.x), float(un.y)};
         }
     }
 }

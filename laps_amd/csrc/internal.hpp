@@ -157,6 +157,7 @@ struct ZArgs {
     const cpx<double>* phc;   // (kx, cy, p)
     const cpx<double>* phb;
     const cpx<double>* clos;  // optional learned-closure RHS [6][modes] added to momentum and B (v = 1..6)
+    cpx<float>* Uf;           // optional FP32 copy of the updated state (mixed precision)
 };
 template <typename T>
 hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,
@@ -169,6 +170,8 @@ hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, 
 
 // --- physics launchers (physics_kernels.hip) ---
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fstride, Box b, BoxTables t,
+                       const PhysParams& pp, const Ctl* ctl, hipStream_t s);
+hipError_t launch_curl(const cpx<double>* B, cpx<float>* J, long long fstride, Box b, BoxTables t,
                        const PhysParams& pp, const Ctl* ctl, hipStream_t s);
 hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s);
 hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s);

@@ -141,6 +141,9 @@ struct laps_ctx {
     long long p1 = 0, p2 = 0;                             // complex elements per chain of P1, P2
     // learned closure: parameters + normalisation (FP32), features, outputs, the 9 real
     // correction fields, their spectra, and the RHS term on the full and the kept box
+    // mixed precision: FP32 copy of U (8 state fields written by the z pass, 3 Hall-current
+    // fields by the curl), the z-inverse input; U stays the FP64 state
+    cpx<float>* Uf = nullptr;
     bool closure = false;
     float *cl_P = nullptr, *cl_X = nullptr, *cl_Y = nullptr, *cl_id = nullptr;
     double* cl_C = nullptr;
@@ -352,8 +355,8 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 // axes at once, as the modes are independent of each other in the y and x legs.
 // With nch = 2 the z leg also writes the shifted fields as chain 1 (at A + a_ch),
 // read once from src, and the y leg transforms both chains (chain 1 to B + b_ch).
-template <typename T>
-int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
+template <typename T, typename TS = double>
+int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
                  cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false, int nch = 1, long long a_ch = 0,
                  long long b_ch = 0) {
     if (nch == 2 && (dst || !shift)) return fail(LAPS_EINVAL, "internal: two chains are the shifted pair");
@@ -371,8 +374,8 @@ int inv_pipeline(laps_ctx* c, const cpx<double>* src, long long fs, const Box& b
     z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * RS;
     z.nch = nch; z.out_ch = a_ch; z.pair = true;
     const double ez = (double)nf * b.CY * b.KX;
-    LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 16.0 + nch * NZ * 2.0 * sizeof(T)),
-                  [&] { return launch_strided<double, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
+    LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 2.0 * sizeof(TS) + nch * NZ * 2.0 * sizeof(T)),
+                  [&] { return launch_strided<TS, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
     y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
     y.in_ls = RS; y.out_ls = RS;
@@ -423,6 +426,15 @@ int fwd_pipeline(laps_ctx* c, const T* src, int nf, const Box& b, cpx<T>* A, cpx
     return LAPS_OK;
 }
 
+// The z-inverse input of a substage: the FP64 state, or its FP32 copy in mixed precision
+// (the transforms round their input to FP32 at load either way).
+template <typename T>
+const cpx<T>* state_src(laps_ctx* c);
+template <>
+const cpx<double>* state_src<double>(laps_ctx* c) { return c->U; }
+template <>
+const cpx<float>* state_src<float>(laps_ctx* c) { return c->Uf; }
+
 // one substage of evolve (mhd.f90:303-325): [curl J] -> z, y inverse ->
 // fused x pass (C2R, fluxes, CFL, R2C) -> y, z forward -> RHS + RK3 + dealias
 template <typename T>
@@ -464,11 +476,13 @@ int substage(laps_ctx* c, int irk) {
         xin = 96.0 * c->N + (hall ? 3.0 * xa.nrows * c->nxh * 2 * sizeof(T) : 0.0);
     } else {
         if (hall)
-            LAPS_CK(timed(c, K_CURL, 96.0 * KB, [&] {
+            LAPS_CK(timed(c, K_CURL, (48.0 + 6.0 * sizeof(T)) * KB, [&] {
+                if (c->Uf)
+                    return launch_curl(c->U, c->Uf + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
                 return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
             }));
-        if ((rc = inv_pipeline<T>(c, c->U, KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv, c->plan_yinv,
-                                  nch == 2, nch, c->p1, c->p2)))
+        if ((rc = inv_pipeline<T, T>(c, state_src<T>(c), KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv,
+                                     c->plan_yinv, nch == 2, nch, c->p1, c->p2)))
             return rc;
         xa.nch = nch;
         xa.st_ch = c->p2;
@@ -528,6 +542,7 @@ int substage(laps_ctx* c, int irk) {
         }));
     }
     za.clos = c->closure ? c->cl_rhs : nullptr;
+    za.Uf = c->Uf;
     if (c->p.dealias_option == 3) {
         // the z pass averages the right-hand sides of the two grids
         za.Ds = P3;
@@ -537,7 +552,8 @@ int substage(laps_ctx* c, int irk) {
     }
     const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
     // U read + write (FP64), Fp write (+ read when irk > 0) in the transform type
-    const double zb = c->zfwd_bytes * sizeof(T) * (za.Ds ? 2 : 1) + 8.0 * KB * (32.0 + 2.0 * sizeof(T) * (irk ? 2 : 1));
+    const double zb = c->zfwd_bytes * sizeof(T) * (za.Ds ? 2 : 1) + 8.0 * KB * (32.0 + 2.0 * sizeof(T) * (irk ? 2 : 1)) +
+                      (c->Uf ? 8.0 * KB * 8.0 : 0.0);
     LAPS_CK(timed(c, K_ZFWD, zb, [&] {
         return launch_zfwd_rhs<T>(c->nz, P2, c->U, (cpx<T>*)c->Fp, za, tw_of<T>(c, 2), c->tkept.view(), c->tfull.wz, c->pp,
                                   c->ctl, irk, scale, c->stream);
@@ -570,7 +586,7 @@ int autotune(laps_ctx* c) {
         for (int rep = 0; rep < 4; ++rep) {
             // z+y with plan for the y part held at 0, then y alone with plan
             LAPS_CK(hipEventRecord(e0, c->stream));
-            int rc = inv_pipeline<T>(c, c->U, c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, plan, 0);
+            int rc = inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, plan, 0);
             if (rc) return rc;
             LAPS_CK(hipEventRecord(e1, c->stream));
             LAPS_CK(hipEventSynchronize(e1));
@@ -961,8 +977,10 @@ int laps_create(const laps_params* p, laps_ctx** out) {
         (rc = c->alloc(&c->P1, nchain * p1 * 2 * ts)) || (rc = c->alloc(&c->P2, nchain * p2 * 2 * ts)) ||
         (rc = c->allocn(&c->X1, xs)) || (rc = c->allocn(&c->X2, xs)) || (rc = c->allocn(&c->Xr, c->N)) ||
         (rc = c->allocn(&c->red, (size_t)c->nblk_red * 16)) || (rc = c->allocn(&c->dave, 8)) ||
-        (rc = c->allocn(&c->dbits, 1)))
+        (rc = c->allocn(&c->dbits, 1)) || (c->mixed && (rc = c->allocn(&c->Uf, 11 * KB))))
         return bail(rc);
+    if (c->Uf && hipMemset(c->Uf, 0, 11 * KB * sizeof(cpx<float>)) != hipSuccess)
+        return bail(fail(LAPS_EDEVICE, "clearing the FP32 state copy failed"));
     if (shift) {
         // half a cell along every axis: e^{i k delta} = e^{i pi m/n}, signed m with the +Nyquist
         // convention of the wave numbers
@@ -1012,6 +1030,8 @@ int laps_set_state(laps_ctx* c, const double* uu, const double* uu_prim) {
     LAPS_CK(hipMemsetAsync(c->U, 0, 11 * c->kb() * sizeof(cpx<double>), c->stream));
     LAPS_CK(hipMemsetAsync(c->Fp, 0, 8 * c->kb() * sizeof(cpx<double>), c->stream));
     LAPS_CK(launch_gather_box(c->Ufull, c->U, 8, c->M, c->kb(), c->nxh, c->ny, c->nz, c->kept, c->stream));
+    if (c->Uf)
+        LAPS_CK(launch_convert<double, float>((const double*)c->U, (float*)c->Uf, 2 * 8 * c->kb(), c->stream));
     if ((rc = reset_ctl(c, 0.0, 0.0, false))) return rc;
     LAPS_CK(hipStreamSynchronize(c->stream));
     c->have_state = true;

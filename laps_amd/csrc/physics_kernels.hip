@@ -50,7 +50,9 @@ __device__ __forceinline__ bool box_index(long long idx, const Box& b, int& kx, 
 }
 
 // ------------------------------------------------------------------ curl
-__global__ void k_curl(const cpx<double>* __restrict__ U, cpx<double>* __restrict__ J, long long fs, Box b,
+// J is written in FP64, or rounded once to FP32 for the mixed-precision step's z-inverse input
+template <typename TJ>
+__global__ void k_curl(const cpx<double>* __restrict__ U, cpx<TJ>* __restrict__ J, long long fs, Box b,
                        BoxTables t, PhysParams pp, const Ctl* __restrict__ ctl) {
     const long long n = b.modes();
     const double radius = ctl->radius;
@@ -59,15 +61,23 @@ __global__ void k_curl(const cpx<double>* __restrict__ U, cpx<double>* __restric
         if (!box_index(i, b, kx, cy, cz)) continue;
         const KVec k = kvec(t.wx[kx], t.wy[cy], t.wz[cz], pp, radius);
         const cpx<double> bx = U[4 * fs + i], by = U[5 * fs + i], bz = U[6 * fs + i];
-        J[i] = ik(k.ky, bz) - ik(k.kz, by);
-        J[fs + i] = ik(k.kz, bx) - ik(k.kx, bz);
-        J[2 * fs + i] = ik(k.kx, by) - ik(k.ky, bx);
+        const cpx<double> jx = ik(k.ky, bz) - ik(k.kz, by), jy = ik(k.kz, bx) - ik(k.kx, bz);
+        const cpx<double> jz = ik(k.kx, by) - ik(k.ky, bx);
+        J[i] = cpx<TJ>{TJ(jx.x), TJ(jx.y)};
+        J[fs + i] = cpx<TJ>{TJ(jy.x), TJ(jy.y)};
+        J[2 * fs + i] = cpx<TJ>{TJ(jz.x), TJ(jz.y)};
     }
 }
 
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fs, Box b, BoxTables t, const PhysParams& pp,
                        const Ctl* ctl, hipStream_t s) {
-    hipLaunchKernelGGL(k_curl, dim3(grid_for(b.modes())), dim3(kThreads), 0, s, B, J, fs, b, t, pp, ctl);
+    hipLaunchKernelGGL(k_curl<double>, dim3(grid_for(b.modes())), dim3(kThreads), 0, s, B, J, fs, b, t, pp, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_curl(const cpx<double>* B, cpx<float>* J, long long fs, Box b, BoxTables t, const PhysParams& pp,
+                       const Ctl* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_curl<float>, dim3(grid_for(b.modes())), dim3(kThreads), 0, s, B, J, fs, b, t, pp, ctl);
     return hipGetLastError();
 }
 
