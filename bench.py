@@ -220,6 +220,90 @@ def run_ensemble(steps):
                 time_range=[min(x["time"] for x in g["scalars"]), max(x["time"] for x in g["scalars"])])
 
 
+def config2_params(precision):
+    """512^3 mixed-precision Hall-MHD with the corotating expanding box: R0 = 1, Ur0 = 0.5 (Ur0/R0 = 0.5
+    in box units), d_i = 0.02, nu = eta = 5e-5, 2/3 rule, CFL 0.3 (the Hall whistler at the 2/3 rule's
+    k dx = 2.09 turns at cfl (k dx)^2 per step; 0.4 would sit at RK3's limit sqrt(3))."""
+    from laps_amd.params import default_params
+    L = 2 * math.pi
+    return default_params(nx=512, ny=512, nz=512, Lx=L, Ly=L, Lz=L, adiabatic_index=5.0 / 3.0, cfl=0.3,
+                          dealias_option=1, if_visc=1, viscosity=5e-5, if_resis=1, resistivity=5e-5, if_hall=1,
+                          ion_inertial_length=0.02, if_aeb=1, if_corotating=1, radius0=1.0, Ur0=0.5,
+                          precision=precision)
+
+
+def config2_state(seed=2):
+    """Outward Alfven-wave spectrum, amplitude k^-1 in shells 1-32, db/B0 = 0.5, du = -db/sqrt(rho),
+    rho = p = 1, B0 = (1, 0, 0). The field has no mode beyond shell 32, so it is generated on 128^3 (same
+    seeded modes, same rms) and brought to 512^3 by the library's own inverse transform of its spectrum."""
+    from laps_amd.ic import cons_from_prim, seeded_primitive
+    from laps_amd.params import LAPS_FP64
+    from laps_amd.runtime import Laps
+    m, n = 128, 512
+    pc = seeded_primitive((m, m, m), seed, kmin=1, kmax=32, slope=-1.0, du=0.5, db=0.5, alfvenic=True)
+    idx = np.arange(m)
+    idx = np.where(idx <= m // 2, idx, idx - m) % n
+    spec = np.zeros((n // 2 + 1, n, n, 6), dtype=np.complex128, order="F")
+    for f, (col, off) in enumerate(((1, 0.0), (2, 0.0), (3, 0.0), (4, 1.0), (5, 0.0), (6, 0.0))):
+        F = np.fft.rfftn(pc[..., col] - off, axes=(2, 1, 0)) / m ** 3
+        spec[np.ix_(np.arange(m // 2 + 1), idx, idx, [f])] = F[..., None]
+    del pc
+    with Laps(config2_params(LAPS_FP64)) as g:
+        real = g.fft_c2r(spec)
+    del spec
+    prim = np.zeros((n, n, n, 8), order="F")
+    prim[..., 0] = 1.0
+    prim[..., 7] = 1.0
+    prim[..., 1:4] = real[..., 0:3]
+    prim[..., 4:7] = real[..., 3:6]
+    prim[..., 4] += 1.0
+    del real
+    return cons_from_prim(prim, 5.0 / 3.0)
+
+
+def run_config2(steps):
+    """FP64 and mixed runs of the 512^3 expanding-box workload: steps/s of each, and the mixed run's
+    relative L2 distance to the FP64 run after `steps` steps at equal time levels."""
+    from laps_amd.params import LAPS_FP64, LAPS_MIXED
+    from laps_amd.runtime import Laps
+    t0 = time.perf_counter()
+    uu, pr = config2_state()
+    log(f"config 2: 512^3 state built in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    dts = []
+    for prec in (LAPS_FP64, LAPS_MIXED):
+        key = "fp64" if prec == LAPS_FP64 else "mixed"
+        with Laps(config2_params(prec)) as g:
+            g.set_state(uu, pr)
+            g.step(1)   # first step: plan autotune; not timed
+            g.sync()
+            if prec == LAPS_FP64:
+                dts.append(g.scalars()["dt"])
+                t1 = time.perf_counter()
+                for _ in range(steps - 1):
+                    g.step(1)
+                    dts.append(g.scalars()["dt"])
+            else:
+                t1 = time.perf_counter()
+                for dt in dts[1:]:
+                    g.set_fixed_dt(dt)
+                    g.step(1)
+            g.sync()
+            sec = time.perf_counter() - t1
+            sc = g.scalars()
+            out[key] = dict(steps_per_s=round((steps - 1) / sec, 3), time=sc["time"], radius=sc["radius"],
+                            max_divb=g.max_divb(), spectral=g.spectral())
+            log(f"config 2 {key}: {steps - 1} steps in {sec:.2f} s, t={sc['time']:.5f}, R={sc['radius']:.5f}")
+    a, b = out["mixed"].pop("spectral"), out["fp64"].pop("spectral")
+    rel = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+    return dict(workload="512^3 Hall-MHD, corotating expanding box (R0=1, Ur0=0.5), d_i=0.02, nu=eta=5e-5, "
+                         "outward Alfven k^-1 shells 1-32, db/B0=0.5, 2/3 rule, CFL 0.3",
+                steps=steps, fp64=out["fp64"], mixed=out["mixed"], mixed_rel_l2_vs_fp64=rel,
+                compared_at="the mixed run replays the FP64 run's CFL dt sequence (timed with the per-step "
+                            "host reads of dt and set_fixed_dt)",
+                dt_range=[min(dts), max(dts)])
+
+
 def roofline(timing, precision):
     name, (ms, by, nl) = max(timing.items(), key=lambda kv: kv[1][0])
     per_launch_s = ms * 1e-3 / max(nl, 1)
@@ -271,6 +355,8 @@ def main():
                     help="steps of the 96-box ensemble reported under 'ensemble' (0: skip it)")
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the learned-closure experiment reported under 'closure'")
+    ap.add_argument("--config2", type=int, default=0, metavar="STEPS",
+                    help="also run the 512^3 expanding-box workload for STEPS steps (under 'config2')")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run y = 2x over 1 GiB (1 GiB read + 1 GiB written) for HBM counter calibration")
@@ -378,6 +464,8 @@ def main():
     if not args.no_closure:
         from laps_amd.closure import experiment
         out["closure"] = experiment(log=log)
+    if args.config2 > 0:
+        out["config2"] = run_config2(args.config2)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         try:
