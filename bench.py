@@ -362,7 +362,7 @@ def main():
                     help="first run y = 2x over 1 GiB (1 GiB read + 1 GiB written) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+        print("bench.py runs on exactly one GPU: there is no multi-GPU path "
               "(the box would shard as slabs with one all-to-all per 3D transform; not built).",
               file=sys.stderr)
         sys.exit(2)

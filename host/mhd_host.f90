@@ -52,7 +52,7 @@ program mhd_host
     ! &Hall
     logical :: if_hall = .false.
     real(dp) :: ion_inertial_length = 0
-    ! &laps (optional, this build only)
+    ! &laps (optional; no counterpart in the reference)
     integer :: nstep = -1, device = 0
     character(len=8) :: precision = 'fp64'
 

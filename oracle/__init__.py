@@ -1,5 +1,5 @@
 """CPU oracle of the LAPS time step (test infrastructure only).
 
 Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
-as the checker; the product package laps_amd never imports it.
+as the reference results; the product package laps_amd never imports it.
 """

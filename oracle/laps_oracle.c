@@ -2,7 +2,7 @@
  * laps_oracle.c -- CPU restatement (plain C, FP64, OpenMP over pencils) of the
  * reference LAPS compressible Hall-MHD RK3 time step.
  *
- * TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP product path:
+ * TEST INFRASTRUCTURE ONLY. This file gives the reference results for the HIP path:
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
  * Nothing in laps_amd/ links, loads or calls it.
  *

@@ -1,6 +1,6 @@
 /*
  * laps_oracle.h -- CPU restatement (plain C, FP64) of the reference LAPS
- * compressible Hall-MHD time step. TEST INFRASTRUCTURE ONLY: the checker used
+ * compressible Hall-MHD time step. TEST INFRASTRUCTURE ONLY: the reference used
  * by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py.
  * The product library (laps_amd/liblaps_amd.so) never links or calls it.
  *

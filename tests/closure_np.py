@@ -1,4 +1,4 @@
-"""numpy restatement of the learned closure (the checker of the HIP kernels in
+"""numpy restatement of the learned closure (the test reference for the HIP kernels in
 laps_amd/csrc/closure.hip): network forward and mean-squared-error gradient in
 float64, the 28 features, the correction fields and their spectral RHS term."""
 import numpy as np

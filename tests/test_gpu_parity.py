@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle.
 
 FP64: relative L2 of the spectral state <= 1e-11 after N steps (the bound
-of the FP64 north-star target); mixed precision: <= 1e-4 against the FP64
+of the FP64 target); mixed precision: <= 1e-4 against the FP64
 oracle. Sizes are those the oracle finishes in seconds.
 """
 import math
