@@ -44,10 +44,16 @@ struct XArgs {
 };
 
 // Device control block of one box (scalars of mhd.f90 / AEBmod.f90).
+// The CFL minimum of a step is folded by 64-bit atomicMin on the bit patterns
+// of non-negative doubles, spread over kCflSlots addresses (work-group index
+// modulo kCflSlots) so that the waves of a pass do not queue on one address;
+// k_ctl_finalize folds the slots.
+constexpr int kCflSlots = 256;
 struct Ctl {
     double time, dt, dt_prev, radius, tau, r0r;
     double cc[3], dd[3], ts[3];
     unsigned long long dtmin_bits;
+    unsigned long long dtmin_slot[kCflSlots];
     long long istep;
     int status;
     int ksq_updated;

@@ -835,6 +835,7 @@ int reset_ctl(laps_ctx* c, double time, double dt_prev, bool ksq_updated) {
     h.tau = h.radius / c->pp.Ur;
     h.r0r = c->pp.radius0 / h.radius;
     h.dtmin_bits = 0x7FF0000000000000ULL;
+    for (int i = 0; i < kCflSlots; ++i) h.dtmin_slot[i] = 0x7FF0000000000000ULL;
     h.istep = 0;
     h.status = 0;
     h.ksq_updated = ksq_updated ? 1 : 0;

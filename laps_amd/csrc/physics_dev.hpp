@@ -13,32 +13,37 @@ __device__ __forceinline__ R cfl_div(R a, R b) {
     else return a / b;
 }
 
+// square root in the arithmetic R of cfl_point: correctly rounded in FP64; the
+// hardware square root (about 1 ulp) in FP32; x >= 0 here
+template <typename R>
+__device__ __forceinline__ R cfl_sqrt(R x) {
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_sqrtf(x);
+    else return sqrt(x);
+}
+
 // dt candidate of one point (vardt, mhd.f90:355-409). R = double follows the
 // reference's FP64 arithmetic; the mixed-precision step evaluates it in FP32
-// (R = float), where the fields themselves are FP32.
+// (R = float), where the fields themselves are FP32. The reference takes per
+// axis the largest of |u +- c_f|, |u +- c_s|, |u +- c_a| and |u|; with
+// c_f >= c_a, c_s >= 0 (the fast speed bounds the others) that is |u| + c_f,
+// which is what is computed here (equal up to the rounding of c_f against c_a
+// in degenerate states, a few ulp of dt).
 template <typename R>
 __device__ __forceinline__ R cfl_point(R rho, R Bx, R By, R Bz, R p, R ux, R uy, R uz, const PhysParams& pp, R rr,
                                        R dmin) {
     const R cs2 = cfl_div(R(pp.gamma) * p, rho);
-    const R sr = sqrt(rho);
+    const R sr = cfl_sqrt(rho);
     const R cax = cfl_div(Bx, sr), cay = cfl_div(By, sr), caz = cfl_div(Bz, sr);
     const R ca2 = cax * cax + cay * cay + caz * caz;
     const R cms2 = cs2 + ca2;
     const R zero = R(0);
-    const R cnx = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cax * cax, zero));
-    const R cny = sqrt(fmax(cms2 * cms2 - 4 * cs2 * cay * cay, zero));
-    const R cnz = sqrt(fmax(cms2 * cms2 - 4 * cs2 * caz * caz, zero));
-    const R s2 = sqrt(R(2));
-    const R cfx = cfl_div(sqrt(cms2 + cnx), s2), cfy = cfl_div(sqrt(cms2 + cny), s2);
-    const R cfz = cfl_div(sqrt(cms2 + cnz), s2);
-    const R csx = cfl_div(sqrt(fmax(cms2 - cnx, zero)), s2);
-    const R csy = cfl_div(sqrt(fmax(cms2 - cny, zero)), s2);
-    const R csz = cfl_div(sqrt(fmax(cms2 - cnz, zero)), s2);
-    auto cmax = [](R u, R cf, R cs, R ca) {
-        return fmax(fmax(fmax(fabs(u + cf), fabs(u + cs)), fmax(fabs(u + ca), fabs(u - cf))),
-                    fmax(fmax(fabs(u - cs), fabs(u - ca)), fabs(u)));
-    };
-    R cmx = cmax(ux, cfx, csx, cax), cmy = cmax(uy, cfy, csy, cay), cmz = cmax(uz, cfz, csz, caz);
+    const R cnx = cfl_sqrt(fmax(cms2 * cms2 - 4 * cs2 * cax * cax, zero));
+    const R cny = cfl_sqrt(fmax(cms2 * cms2 - 4 * cs2 * cay * cay, zero));
+    const R cnz = cfl_sqrt(fmax(cms2 * cms2 - 4 * cs2 * caz * caz, zero));
+    const R s2 = cfl_sqrt(R(2));
+    const R cfx = cfl_div(cfl_sqrt(cms2 + cnx), s2), cfy = cfl_div(cfl_sqrt(cms2 + cny), s2);
+    const R cfz = cfl_div(cfl_sqrt(cms2 + cnz), s2);
+    R cmx = fabs(ux) + cfx, cmy = fabs(uy) + cfy, cmz = fabs(uz) + cfz;
     if (pp.hall) {
         R bmax = Bx;  // signed max of the components (mhd.f90:396-398)
         if (By > bmax) bmax = By;

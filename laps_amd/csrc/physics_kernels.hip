@@ -99,7 +99,8 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* b
 
 // ------------------------------------------------------------------ control block
 __global__ void k_ctl_begin(Ctl* ctl) {
-    ctl->dtmin_bits = 0x7FF0000000000000ULL;  // +Inf
+    ctl->dtmin_slot[threadIdx.x] = 0x7FF0000000000000ULL;  // +Inf
+    if (threadIdx.x == 0) ctl->dtmin_bits = 0x7FF0000000000000ULL;
 }
 
 // vardt tail + rkt_init + the dt < 1e-8 stop test (mhd.f90:216, :418-428; rktmod.f90:15-32)
@@ -109,7 +110,10 @@ __global__ void k_ctl_finalize(Ctl* ctl, PhysParams pp) {
     if (pp.fixed_dt > 0) {
         dt = pp.fixed_dt;
     } else {
-        const double dtmin = __longlong_as_double((long long)ctl->dtmin_bits) * pp.cfl;
+        unsigned long long m = ctl->dtmin_bits;
+        for (int i = 0; i < kCflSlots; ++i) m = ctl->dtmin_slot[i] < m ? ctl->dtmin_slot[i] : m;
+        ctl->dtmin_bits = m;
+        const double dtmin = __longlong_as_double((long long)m) * pp.cfl;
         dt = ctl->dt_prev;
         if (dt < 0.98 * dtmin || dt > 1.02 * dtmin) dt = dtmin;
     }
@@ -142,7 +146,7 @@ __global__ void k_ctl_end(Ctl* ctl, PhysParams pp) {
 }
 
 hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s) {
-    hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(1), 0, s, ctl);
+    hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(kCflSlots), 0, s, ctl);
     return hipGetLastError();
 }
 hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s) {

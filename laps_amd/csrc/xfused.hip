@@ -386,7 +386,9 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
             }
         }
         for (int o = 32; o > 0; o >>= 1) dtl = fmin(dtl, __shfl_xor(dtl, o));
-        if ((threadIdx.x & 63) == 0) atomicMin(&ctl->dtmin_bits, (unsigned long long)__double_as_longlong(dtl));
+        if ((threadIdx.x & 63) == 0)
+            atomicMin(&ctl->dtmin_slot[(blockIdx.x * (NT / 64) + threadIdx.x / 64) % kCflSlots],
+                      (unsigned long long)__double_as_longlong(dtl));
     }
 
     if constexpr (HALL) {
