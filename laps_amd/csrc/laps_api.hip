@@ -582,11 +582,14 @@ int autotune(laps_ctx* c) {
     LAPS_CK(hipEventCreate(&e0));
     LAPS_CK(hipEventCreate(&e1));
     float best[2][2] = {{0, 0}, {0, 0}};  // [pass][plan]
+    // the launches a step makes: with phase-shift dealiasing both grids as two chains
+    const int nch = c->p.dealias_option == 3 ? 2 : 1;
     for (int plan = 0; plan < 2; ++plan) {
         for (int rep = 0; rep < 4; ++rep) {
             // z+y with plan for the y part held at 0, then y alone with plan
             LAPS_CK(hipEventRecord(e0, c->stream));
-            int rc = inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, plan, 0);
+            int rc = inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, plan, 0,
+                                        nch == 2, nch, c->p1, c->p2);
             if (rc) return rc;
             LAPS_CK(hipEventRecord(e1, c->stream));
             LAPS_CK(hipEventSynchronize(e1));
@@ -598,6 +601,7 @@ int autotune(laps_ctx* c) {
             y.in_ls = RS; y.out_ls = RS;
             y.nb1 = (int)NZ; y.in_b1 = (long long)k.CY * RS; y.out_b1 = NY * RS;
             y.nb2 = c->nf_in(); y.in_b2 = NZ * k.CY * RS; y.out_b2 = NZ * NY * RS;
+            y.nch = nch; y.in_ch = c->p1; y.out_ch = c->p2;
             LAPS_CK(hipEventRecord(e0, c->stream));
             LAPS_CK(launch_strided<T, T>(c->ny, +1, P1, P2, y, tw_of<T>(c, 1), T(1), c->stream, plan));
             LAPS_CK(hipEventRecord(e1, c->stream));
