@@ -21,6 +21,7 @@ CASES = [
     dict(n=32, seed=13, dealias_option=3, if_hall=1, ion_inertial_length=0.05, cfl=0.2),
     dict(n=(64, 32, 32), seed=14, if_aeb=1, radius0=1.0, Ur0=0.5, Lx=9.0),
     dict(n=32, seed=15, precision=LAPS_MIXED, dealias_option=2),
+    dict(n=(96, 32, 32), seed=16, precision=LAPS_MIXED, if_hall=1, ion_inertial_length=0.05),  # radix 3
 ]
 
 
@@ -86,7 +87,7 @@ def test_graph_replays_match_eager_steps(rt):
             s.step(3)
         check_equal(ens, solo, "after re-capture")
         steps = [ens.box(i).scalars()["istep"] for i in range(len(ens))]
-        assert steps == [12, 3, 12, 12, 12]
+        assert steps == [12, 3, 12, 12, 12, 12]
     finally:
         ens.close()
         for s in solo:
