@@ -107,6 +107,46 @@ module laps_c
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: out
         end function
+
+        ! ensembles of independent boxes (no counterpart in the reference): params is an
+        ! array of nbox laps_params; laps_ens_box returns box i's context
+        integer(c_int) function laps_ens_create(nbox, params, ens) bind(C, name='laps_ens_create')
+            import :: c_int, c_ptr, laps_params
+            integer(c_int), value :: nbox
+            type(laps_params), intent(in) :: params(*)
+            type(c_ptr), intent(out) :: ens
+        end function
+
+        subroutine laps_ens_destroy(ens) bind(C, name='laps_ens_destroy')
+            import :: c_ptr
+            type(c_ptr), value :: ens
+        end subroutine
+
+        type(c_ptr) function laps_ens_box(ens, i) bind(C, name='laps_ens_box')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ens
+            integer(c_int), value :: i
+        end function
+
+        integer(c_int) function laps_ens_step(ens, nsteps) bind(C, name='laps_ens_step')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ens
+            integer(c_int), value :: nsteps
+        end function
+
+        integer(c_int) function laps_ens_sync(ens) bind(C, name='laps_ens_sync')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ens
+        end function
+
+        ! learned closure (no counterpart in the reference): params holds
+        ! LAPS_CLOSURE_NPARAM = 7062 reals; a null pointer switches it off
+        integer(c_int) function laps_closure_set(ctx, params, nparams) bind(C, name='laps_closure_set')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            type(c_ptr), value :: params
+            integer(c_int), value :: nparams
+        end function
     end interface
 
 contains
