@@ -615,8 +615,10 @@ int autotune(laps_ctx* c) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // plan 1 only when clearly faster (3 %), so repeated runs pick the same code
-    c->plan_zinv = best[0][1] < 0.97f * best[0][0] ? 1 : 0;
+    // plan 1 only when clearly faster (3 %), so repeated runs pick the same code; the z pass
+    // is compared on its own time (z + y with the y plan held at 0, less that y time)
+    const float z0 = best[0][0] - best[1][0], z1 = best[0][1] - best[1][0];
+    c->plan_zinv = z1 < 0.97f * z0 ? 1 : 0;
     c->plan_yinv = best[1][1] < 0.97f * best[1][0] ? 1 : 0;
     c->tuned = true;
     return LAPS_OK;
