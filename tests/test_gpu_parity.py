@@ -181,7 +181,9 @@ def test_config0_64cube_100_steps(laps_mod, oracle_mod):
 
 
 @pytest.mark.parametrize("case", [dict(), dict(if_hall=1, ion_inertial_length=0.05), dict(dealias_option=2),
-                                  dict(dealias_option=3, if_hall=1, ion_inertial_length=0.05)])
+                                  dict(dealias_option=3, if_hall=1, ion_inertial_length=0.05),
+                                  dict(if_aeb=1, if_corotating=1, corotating_angle=0.3, radius0=1.0, Ur0=0.5,
+                                       if_hall=1, ion_inertial_length=0.02)])
 def test_mixed_precision_vs_fp64_oracle(laps_mod, oracle_mod, case):
     p = params(32, precision=LAPS_MIXED, **case)
     g = laps_mod.Laps(p)
