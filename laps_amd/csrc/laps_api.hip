@@ -16,7 +16,7 @@
 // The "kept box" holds kx < KX and the stored ky/kz rows of the modes that
 // survive dealiasing: all modes for the compact filter (option 2), the
 // bounding box of the 2/3 sphere for option 1. Modes outside the box are
-// identically zero after every substage (dealiasing.f90:152-164), so the
+// identically zero after every substage (dealiasing.f90:87-99), so the
 // transforms read and write only the box.
 #include <cmath>
 #include <cstdio>
@@ -83,7 +83,7 @@ double wave_number(int i, int n, double L) {
 
 int signed_index(int i, int n) { return (i <= n / 2) ? i : i - n; }
 
-// dealiasing.f90:101-108
+// dealiasing.f90:100-111
 double compact_filter(double w, double L, int n, double af) {
     const double aj = (5.0 + 6.0 * af) / 8.0;
     const double bj = (1.0 + 2.0 * af) / 2.0;
@@ -227,7 +227,7 @@ int upload_tables(laps_ctx* c, const Box& b, bool pruned, const std::vector<int>
     return LAPS_OK;
 }
 
-// Bounding box of the modes kept by the 2/3 rule (dealiasing.f90:152-164),
+// Bounding box of the modes kept by the 2/3 rule (dealiasing.f90:87-99),
 // evaluated with the reference's expression, or by phase-shift dealiasing
 // (option 3, no reference counterpart: the open ellipsoid
 // (mx/nx)^2 + (my/ny)^2 + (mz/nz)^2 < 2/9, tested exactly in integers), plus

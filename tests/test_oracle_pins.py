@@ -10,7 +10,9 @@ import math
 import numpy as np
 import pytest
 
-from tests.helpers import params, rel_l2, seeded_state, TWO_PI
+from laps_amd.ic import cons_from_prim
+from tests.helpers import (circular_wave, params, rel_l2, rk3_amplification, seeded_state, TWO_PI,
+                           wave_index)
 
 
 def brute_dft(x, sign):
@@ -69,7 +71,7 @@ def test_single_mode_transform(oracle_mod):
 
 
 def test_dealias_sphere_count(oracle_mod):
-    # option 1 keeps sqrt((mx/nx)^2+(my/ny)^2+(mz/nz)^2) <= 1/3 (dealiasing.f90:152-164)
+    # option 1 keeps sqrt((mx/nx)^2+(my/ny)^2+(mz/nz)^2) <= 1/3 (dealiasing.f90:8, :87-99)
     p = params(64)
     o = oracle_mod.Oracle(p)
     u = np.ones((33, 64, 64, 8), dtype=complex, order="F")
@@ -273,3 +275,34 @@ def test_divb_bruteforce(oracle_mod):
     kx = np.arange(9)[:, None, None]
     d = 1j * (kx * s[..., 4] + m[None, :, None] * s[..., 5] + m[None, None, :] * s[..., 6])
     assert o.max_divb() == pytest.approx(np.abs(d).max(), rel=1e-13)
+
+
+@pytest.mark.parametrize("di,dealias", [(0.0, 1), (0.3, 1), (0.3, 3), (0.0, 3)])
+def test_circular_wave_closed_form(oracle_mod, di, dealias):
+    # an exact nonlinear solution of compressible (Hall-)MHD: the whole composed step
+    # (transforms, 18 fluxes, Hall EMF, spectral RHS, RK3, dealiasing) must reproduce
+    # rk3_amplification(-i omega dt)^n on its one mode and leave every other mode at rest
+    n, m, dt, nst = 32, (2, 4, 4), 2e-3, 6
+    p = params(n, if_visc=0, if_resis=0, if_hall=1 if di else 0, ion_inertial_length=di,
+               dealias_option=dealias)
+    prim, omega = circular_wave(p, m, 0.05, di)
+    uu, pr = cons_from_prim(prim, p.adiabatic_index)
+    o = oracle_mod.Oracle(p)
+    o.set_state(uu, pr)
+    o.set_fixed_dt(dt)
+    s0 = o.spectral()
+    o.step(nst)
+    s1 = o.spectral()
+    g = rk3_amplification(-1j * omega * dt) ** nst
+    want = np.zeros_like(s0)
+    want[0, 0, 0, :] = s0[0, 0, 0, :]
+    w = wave_index(p, m)
+    want[w][1:7] = s0[w][1:7] * g
+    scale = np.abs(s0[w][1:7]).max()
+    assert np.abs(s1 - want).max() < 1e-13 * scale
+    # the check tells RK3 from the exact exponential, and the Hall branch from the other one
+    assert abs(g - np.exp(-1j * omega * dt * nst)) > 1e-10
+    if di:
+        h = 0.5 * di * math.sqrt(36.0)
+        other = rk3_amplification(-1j * 6.0 * (math.sqrt(1 + h * h) + h) * dt) ** nst
+        assert abs(other - g) > 1e-2
