@@ -88,6 +88,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
     const int kx = valid ? col : 0;
     const double radius = ctl->radius;
     const double wx = wxt[kx];
+    const int ylim = a.ylim ? a.ylim[kx] : N;
     cpx<T>* line = sh + l * C::LS;
     auto lds = [&](int p) { return line + p; };
     const cpx<T>* ib = in + (long long)z * N * a.rs + kx;
@@ -101,7 +102,8 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
     };
     auto store = [&](int f, int p, cpx<T> val) {
         const int s = pos_to_stored(p, N, a.cy);
-        if (valid && s >= 0) ob[f * a.out_fs + s * a.rs] = val;
+        const int my = p <= N / 2 ? p : p - N;
+        if (valid && s >= 0 && my <= ylim && -my <= ylim) ob[f * a.out_fs + s * a.rs] = val;
     };
 
     cpx<T> va[I::RMAX], vb[I::RMAX];
@@ -240,6 +242,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
     const double c = ctl->cc[irk], d = ctl->dd[irk], ts = ctl->ts[irk];
     const double wx = tb.wx[kx], wy = tb.wy[cy];
     const int lim = tb.kzlim[(long long)cy * a.KX + kx];
+    const bool live = valid && (OPT != 1 || lim >= 0);
     cpx<T>* line = sh + l * C::LS;
     auto lds = [&](int p) { return line + p; };
     const cpx<T>* ib = D + (long long)cy * a.rs + kx;
@@ -258,7 +261,8 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
             for (int t = 0; t < 3; ++t) {
                 if (t >= V.nterm) break;
                 const cpx<T>* src = base + V.t[t].field * a.in_fs;
-                auto load = [&](int p) -> cpx<T> { return valid ? src[p * zls] : cpx<T>{T(0), T(0)}; };
+                // a column with no kept mode (OPT 1) is not read: the y pass did not store it
+                auto load = [&](int p) -> cpx<T> { return live ? src[p * zls] : cpx<T>{T(0), T(0)}; };
                 __syncthreads();
                 h = line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, w);
                 if (h) {
@@ -312,6 +316,16 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
             const int cz = pos_to_stored(p, N, a.CZ);
             if (cz < 0) continue;
             const long long mi = (long long)cz * a.CY * a.rs + mode0;
+            if constexpr (OPT == 1) {
+                // outside the kept set the new state is zero whatever U and Fp hold there:
+                // neither is read, and Fp there is never read again
+                const int mz = p <= N / 2 ? p : p - N;
+                if (mz > lim || -mz > lim) {
+                    U[v * a.ufs + mi] = cpx<double>{0.0, 0.0};
+                    if (a.Uf) a.Uf[v * a.ufs + mi] = cpx<float>{0.0f, 0.0f};
+                    continue;
+                }
+            }
             const double wz = swz[p];
             const cpx<T> fv = *lds(p);
             cpx<double> fnl{double(fv.x) * double(scale), double(fv.y) * double(scale)};
@@ -343,10 +357,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
                 const double den = ts * k2 * pp.eta + 1.0;
                 un = {un.x / den, un.y / den};
             }
-            if constexpr (OPT == 1) {
-                const int mz = p <= N / 2 ? p : p - N;
-                if (mz > lim || -mz > lim) un = {0.0, 0.0};
-            } else if constexpr (OPT == 2) {
+            if constexpr (OPT == 2) {
                 const double fx = tb.fx[kx], fy = tb.fy[cy], fz = tb.fz[cz];
                 un = {((un.x * fx) * fy) * fz, ((un.y * fx) * fy) * fz};
             }

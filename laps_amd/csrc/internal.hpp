@@ -74,6 +74,7 @@ struct BoxTables {
     const double* wy;      // [CY]
     const double* wz;      // [CZ]
     const int* kzlim;      // [CY*KX], option 1: keep |mz| <= kzlim (-1: none)
+    const int* ylim;       // [KX]: the rows (kx, my) holding a kept mode are |my| <= ylim
     const double* fx;      // [KX] option 2 filters
     const double* fy;      // [CY]
     const double* fz;      // [CZ]
@@ -143,6 +144,9 @@ struct YGArgs {
     // nch = 2: a second chain at in + in_ch, out + out_ch in the same launch
     int nch;
     long long in_ch, out_ch;
+    // optional [ncol]: output rows (kx, my) with |my| > ylim[kx] hold no kept mode and
+    // are not stored (the z pass does not read them)
+    const int* ylim;
 };
 struct ZTerm {
     int field;

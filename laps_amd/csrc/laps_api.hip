@@ -96,8 +96,8 @@ bool supported_len(int n) { return n == 16 || n == 32 || n == 64 || n == 96 || n
 
 struct DevTables {
     double *wx = nullptr, *wy = nullptr, *wz = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr;
-    int *kzlim = nullptr, *mz = nullptr;
-    BoxTables view() const { return BoxTables{wx, wy, wz, kzlim, fx, fy, fz, mz}; }
+    int *kzlim = nullptr, *mz = nullptr, *ylim = nullptr;
+    BoxTables view() const { return BoxTables{wx, wy, wz, kzlim, ylim, fx, fy, fz, mz}; }
 };
 
 }  // namespace
@@ -164,6 +164,9 @@ struct laps_ctx {
     YGArgs yg{}, yg_aeb{};
     ZArgs zr{}, zr_aeb{};
     double yfwd_bytes = 0, zfwd_bytes = 0;  // algorithmic elements x 2 (multiply by sizeof(T))
+    // modes the dealiasing keeps, and (kx, cy) rows of the kept box holding at least one
+    // (with no mask: every mode and row of the box, padding columns excluded)
+    long long nkept = 0, nlive = 0;
     double fixed_dt = 0.0;
     int nblk_red = 1024;
 
@@ -211,10 +214,17 @@ int upload_tables(laps_ctx* c, const Box& b, bool pruned, const std::vector<int>
     }
     std::vector<int> lim = kzlim;
     if (!pruned) lim.assign((size_t)b.KX * b.CY, c->nz);
+    // per kx, the largest |my| of a row holding a kept mode (-1: none)
+    std::vector<int> ylim(b.KX, -1);
+    for (int s = 0; s < b.CY; ++s) {
+        const int my = std::abs(signed_index(fft_index(s, b.CY, c->ny), c->ny));
+        for (int i = 0; i < b.KX; ++i)
+            if (lim[(size_t)s * b.KX + i] >= 0) ylim[i] = std::max(ylim[i], my);
+    }
     int rc;
     if ((rc = c->allocn(&t.wx, b.KX)) || (rc = c->allocn(&t.fx, b.KX)) || (rc = c->allocn(&t.wy, b.CY)) ||
         (rc = c->allocn(&t.fy, b.CY)) || (rc = c->allocn(&t.wz, b.CZ)) || (rc = c->allocn(&t.fz, b.CZ)) ||
-        (rc = c->allocn(&t.mz, b.CZ)) || (rc = c->allocn(&t.kzlim, lim.size())))
+        (rc = c->allocn(&t.mz, b.CZ)) || (rc = c->allocn(&t.kzlim, lim.size())) || (rc = c->allocn(&t.ylim, b.KX)))
         return rc;
     LAPS_CK(hipMemcpy(t.wx, wx.data(), wx.size() * 8, hipMemcpyHostToDevice));
     LAPS_CK(hipMemcpy(t.fx, fx.data(), fx.size() * 8, hipMemcpyHostToDevice));
@@ -224,6 +234,7 @@ int upload_tables(laps_ctx* c, const Box& b, bool pruned, const std::vector<int>
     LAPS_CK(hipMemcpy(t.fz, fz.data(), fz.size() * 8, hipMemcpyHostToDevice));
     LAPS_CK(hipMemcpy(t.mz, mz.data(), mz.size() * 4, hipMemcpyHostToDevice));
     LAPS_CK(hipMemcpy(t.kzlim, lim.data(), lim.size() * 4, hipMemcpyHostToDevice));
+    LAPS_CK(hipMemcpy(t.ylim, ylim.data(), ylim.size() * 4, hipMemcpyHostToDevice));
     return LAPS_OK;
 }
 
@@ -238,6 +249,8 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
     if (c->p.dealias_option != 1 && !shift) {
         c->kept = c->full;
         c->kept.KXP = (c->full.KX + 15) / 16 * 16;
+        c->nlive = (long long)c->kept.KX * c->kept.CY;
+        c->nkept = c->nlive * c->kept.CZ;
         return LAPS_OK;
     }
     std::vector<double> ax(nxh), ay(ny), az(nz);
@@ -285,6 +298,12 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
     for (int cy = 0; cy < b.CY; ++cy) {
         const int iy = (b.CY == ny || cy <= (b.CY - 1) / 2) ? cy : ny - (b.CY - cy);
         for (int kx = 0; kx < b.KX; ++kx) kzlim[(size_t)cy * b.KX + kx] = col[(size_t)iy * nxh + kx];
+    }
+    c->nkept = c->nlive = 0;
+    for (int lim : kzlim) {
+        if (lim < 0) continue;
+        ++c->nlive;
+        c->nkept += std::min(2 * lim + 1, b.CZ);
     }
     return LAPS_OK;
 }
@@ -511,6 +530,7 @@ int substage(laps_ctx* c, int irk) {
     ya.nch = nch;
     ya.in_ch = c->p1;
     ya.out_ch = c->p2;
+    ya.ylim = c->tkept.ylim;
     LAPS_CK(timed(c, K_YFWD, nch * c->yfwd_bytes * sizeof(T), [&] {
         return launch_yfwd_group<T>(c->ny, P1, P2, ya, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
                                     c->nz, c->stream);
@@ -552,8 +572,11 @@ int substage(laps_ctx* c, int irk) {
     }
     const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
     // U read + write (FP64), Fp write (+ read when irk > 0) in the transform type
-    const double zb = c->zfwd_bytes * sizeof(T) * (za.Ds ? 2 : 1) + 8.0 * KB * (32.0 + 2.0 * sizeof(T) * (irk ? 2 : 1)) +
-                      (c->Uf ? 8.0 * KB * 8.0 : 0.0);
+    // (kept modes only: outside the kept set U is written as zero and neither U nor Fp is read)
+    const double box = (double)k.KX * k.CY * k.CZ;
+    const double zb = c->zfwd_bytes * sizeof(T) * (za.Ds ? 2 : 1) +
+                      8.0 * (16.0 * box + (double)c->nkept * (16.0 + 2.0 * sizeof(T) * (irk ? 2 : 1))) +
+                      (c->Uf ? 8.0 * box * 8.0 : 0.0);
     LAPS_CK(timed(c, K_ZFWD, zb, [&] {
         return launch_zfwd_rhs<T>(c->nz, P2, c->U, (cpx<T>*)c->Fp, za, tw_of<T>(c, 2), c->tkept.view(), c->tfull.wz, c->pp,
                                   c->ctl, irk, scale, c->stream);
@@ -798,9 +821,9 @@ void setup_forward_groups(laps_ctx* c) {
         if (withaeb) { c->yg_aeb = y; c->zr_aeb = z; } else { c->yg = y; c->zr = z; }
     }
     const int nd = c->p.if_aeb ? 16 : 15;
-    const double cols = (double)NZ * k.KX;
-    c->yfwd_bytes = 2.0 * cols * ((double)c->nf_out() * NY + (double)nd * k.CY);
-    c->zfwd_bytes = 2.0 * (double)k.CY * k.KX * nd * NZ;
+    // the y pass stores, and the z pass reads, only the (kx, cy) rows that hold a kept mode
+    c->yfwd_bytes = 2.0 * NZ * ((double)c->nf_out() * NY * k.KX + (double)nd * c->nlive);
+    c->zfwd_bytes = 2.0 * (double)c->nlive * nd * NZ;
 }
 
 PhysParams make_phys(const laps_params& p) {
