@@ -22,15 +22,15 @@
 
 namespace laps {
 
-// ZP: the z pass, whose blocks also carry the RHS and RK3 update. The FP32 y pass at 256 uses
+// ZP: the z pass, whose blocks also carry the RHS and RK3 update. The FP32 passes at 256 use
 // the two-stage radix-16x16 plan (one LDS exchange; FP32 has the registers for 16 values per
-// work-item): 3.09 -> 2.74 ms per phase-shift step. The z pass keeps PlanLow (slower with it).
+// work-item): y 3.09 -> 2.74 ms and z 3.42 -> 3.26 ms per phase-shift step.
 template <typename T, int N, bool ZP>
 struct FwdCfg {
-    using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP), Plan<N>, PlanLow<N>>;
+    using P = std::conditional_t<(sizeof(T) == 4 && N == 256), Plan<N>, PlanLow<N>>;
     using I = PlanInfo<N, P>;
     // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns) in the y pass; the z
-    // pass keeps 8 FP32 columns, whose blocks of 8 waves leave room for more per CU
+    // pass keeps 8 FP32 columns (16 measured slower)
     static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (ZP || N >= 512 ? 8 : 16);
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
