@@ -18,8 +18,8 @@ namespace laps {
 
 template <typename T, int N, class P = Plan<N>>
 struct StridedCfg {
-    static constexpr int TX = (N >= 512) ? (sizeof(T) == 8 ? 4 : 8) : 16;  // 256-byte FP64 rows
     static constexpr int TPL = PlanInfo<N, P>::TPL;
+    static constexpr int TX = (N >= 512) ? (sizeof(T) == 8 ? 4 : 8) : ((sizeof(T) == 4 && TPL <= 16) ? 32 : 16);
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;  // odd line stride: adjacent lines hit different banks
 };

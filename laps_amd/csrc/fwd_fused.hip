@@ -31,7 +31,7 @@ struct FwdCfg {
     using I = PlanInfo<N, P>;
     // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns) in the y pass; the z
     // pass keeps 8 FP32 columns (16 measured slower)
-    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (ZP || N >= 512 ? 8 : 16);
+    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (ZP || N >= 512 ? 8 : (N == 256 ? 32 : 16));
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
@@ -321,8 +321,10 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
                 // neither is read, and Fp there is never read again
                 const int mz = p <= N / 2 ? p : p - N;
                 if (mz > lim || -mz > lim) {
-                    U[v * a.ufs + mi] = cpx<double>{0.0, 0.0};
-                    if (a.Uf) a.Uf[v * a.ufs + mi] = cpx<float>{0.0f, 0.0f};
+                    if (a.clear_outside) {
+                        U[v * a.ufs + mi] = cpx<double>{0.0, 0.0};
+                        if (a.Uf) a.Uf[v * a.ufs + mi] = cpx<float>{0.0f, 0.0f};
+                    }
                     continue;
                 }
             }

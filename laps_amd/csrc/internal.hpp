@@ -168,6 +168,9 @@ struct ZArgs {
     const cpx<double>* phb;
     const cpx<double>* clos;  // optional learned-closure RHS [6][modes] added to momentum and B (v = 1..6)
     cpx<float>* Uf;           // optional FP32 copy of the updated state (mixed precision)
+    // with a mask, write zero to the modes outside the kept set (U, Uf): needed once after a
+    // state upload, which leaves un-dealiased values there; later passes find them zero
+    int clear_outside;
 };
 template <typename T>
 hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,
