@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from laps_amd.params import LAPS_DT_STOP, LAPS_MIXED
-from tests.helpers import params, rel_l2, seeded_state, TWO_PI
+from tests.helpers import kept_modes, params, rel_l2, seeded_state, TWO_PI
 
 pytestmark = pytest.mark.gpu
 
@@ -224,3 +224,36 @@ def test_fixed_dt_and_restart_time(laps_mod, oracle_mod):
     assert g.scalars()["time"] == pytest.approx(0.29, rel=1e-14)
     assert g.scalars()["radius"] == pytest.approx(1.0 + 0.5 * 0.29, rel=1e-14)
     assert rel_l2(g.spectral(), o.spectral()) < FP64_TOL
+
+
+@pytest.mark.parametrize("precision", [0, LAPS_MIXED])
+@pytest.mark.parametrize("dealias", [1, 3])
+def test_state_beyond_the_mask(laps_mod, oracle_mod, dealias, precision):
+    # an uploaded state with energy beyond the dealiasing mask (shells up to 16 at 32^3; the
+    # 2/3 rule keeps |m| <= 10, the phase-shift ellipsoid |m| < 15.1): the first substage
+    # uses it as it is and the step leaves those modes at zero; later substages and steps
+    # rely on finding them zero
+    case = dict(dealias_option=dealias, if_hall=1, ion_inertial_length=0.05,
+                cfl=0.2 if dealias == 3 else 0.4)
+    p = params(32, precision=precision, **case)
+    p64 = params(32, **case)
+    uu, pr = seeded_state(p64, seed=31, kmax=16, du=0.1, db=0.1)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p64)
+    try:
+        g.set_state(uu, pr)
+        o.set_state(uu, pr)
+        s0 = g.spectral()
+        drop = ~kept_modes(p)
+        assert np.abs(s0[drop]).max() > 1e-4   # the upload really holds modes beyond the mask
+        for n in (1, 4):
+            g.step(n)
+            o.step(n)
+            sg = g.spectral()
+            assert not np.any(sg[drop])
+            err = rel_l2(sg, o.spectral())
+            print(f"dealias {dealias} precision {precision} after {n} more steps: rel l2 {err:.2e}")
+            assert err < (FP64_TOL if precision == 0 else MIXED_TOL)
+    finally:
+        g.close()
+        o.close()
