@@ -68,9 +68,18 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
     const long long in_ls = a.in_ls, out_ls = a.out_ls;
     // grid shift: the column and slab factors are fixed per work-item
     const cpx<double> wcb = ph ? cmul(a.phc[valid ? col : 0], a.phb[b1]) : cpx<double>{1.0, 0.0};
+    // z inverse: a row of the mask with no kept mode is zero in and out
+    bool dead = false;
+    if (a.row_role == 1) {
+        const int my = (a.cy_cnt == a.mY) ? (b1 <= a.cy_cnt / 2 ? b1 : b1 - a.cy_cnt)
+                                          : (b1 <= (a.cy_cnt - 1) / 2 ? b1 : b1 - a.cy_cnt);
+        dead = mask_row_dead(a.mask, a.mX, a.mY, a.mZ, col, my);
+    }
     auto load = [&](int p) -> cpx<T> {
         const int s = pos_to_stored(p, N, in_cnt);
-        if (!valid || s < 0) return cpx<T>{T(0), T(0)};
+        if (!valid || s < 0 || dead) return cpx<T>{T(0), T(0)};
+        if (a.row_role == 2 && mask_row_dead(a.mask, a.mX, a.mY, a.mZ, col, p <= N / 2 ? p : p - N))
+            return cpx<T>{T(0), T(0)};
         const cpx<TI> v = ib[s * in_ls];
         if (ph) {
             const cpx<double> w = cmul(ph[p], wcb);
@@ -81,7 +90,7 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
     auto lds = [&](int p) { return line + p; };
     auto store = [&](int p, cpx<T> v) {
         const int s = pos_to_stored(p, N, out_cnt);
-        if (valid && s >= 0) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
+        if (valid && s >= 0 && !dead) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
     };
     cpx<T>* const stp = stw;
     line_fft<N, DIR, T, P>(j, stw, load, lds, store, [twt, stp]() { twt.put(stp); });

@@ -33,7 +33,23 @@ struct StridedArgs {
     int nch;
     long long in_ch, out_ch;
     bool pair;
+    // optional rows of the dealiasing mask (mask 1: 2/3 sphere, 3: phase-shift ellipsoid) for
+    // a kept-box state that is zero outside the kept set: a row (kx = col, my) with no kept
+    // mode even at mz = 0 is all zero. row_role 1 (z inverse, slab b1 = stored ky row of
+    // cy_cnt): such rows are neither loaded nor stored; 2 (y inverse): they are taken as
+    // zero without loading them. The test is integer arithmetic on the grid lengths.
+    int mask, row_role, cy_cnt;
+    long long mX, mY, mZ;
 };
+
+// true when no mode of row (mx, my) is kept: its mz = 0 mode is outside the mask (the
+// sphere test is strict, so rows at exactly r = 1/3 count as kept)
+__host__ __device__ __forceinline__ bool mask_row_dead(int mask, long long X, long long Y, long long Z,
+                                                       long long mx, long long my) {
+    if (mask == 3) return 9 * (mx * mx * Y * Y * Z * Z + my * my * X * X * Z * Z) >= 2 * X * X * Y * Y * Z * Z;
+    if (mask == 1) return 9 * (mx * mx * Y * Y + my * my * X * X) > X * X * Y * Y;
+    return false;
+}
 
 // Rows of an x transform: `nrows` rows (even), complex rows hold `kx` stored
 // coefficients with stride `crow`, real rows hold N values with stride `rrow`.

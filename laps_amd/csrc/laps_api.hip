@@ -377,7 +377,7 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 template <typename T, typename TS = double>
 int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
                  cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false, int nch = 1, long long a_ch = 0,
-                 long long b_ch = 0) {
+                 long long b_ch = 0, bool masked = false) {
     if (nch == 2 && (dst || !shift)) return fail(LAPS_EINVAL, "internal: two chains are the shifted pair");
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
@@ -392,7 +392,16 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
     z.nb1 = b.CY; z.in_b1 = RS; z.out_b1 = RS;
     z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * RS;
     z.nch = nch; z.out_ch = a_ch; z.pair = true;
-    const double ez = (double)nf * b.CY * b.KX;
+    // masked: src is the kept-box state after a masked substage (zero outside the kept set);
+    // the mask rows with no kept mode are skipped by the z inverse and read as zero by the y
+    // inverse
+    const int mask = (masked && (c->p.dealias_option == 1 || c->p.dealias_option == 3)) ? c->p.dealias_option : 0;
+    const double rows = mask ? (double)c->nlive : (double)b.CY * b.KX;
+    if (mask) {
+        z.mask = mask; z.row_role = 1; z.cy_cnt = b.CY;
+        z.mX = NX; z.mY = NY; z.mZ = NZ;
+    }
+    const double ez = (double)nf * rows;
     LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 2.0 * sizeof(TS) + nch * NZ * 2.0 * sizeof(T)),
                   [&] { return launch_strided<TS, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
@@ -401,8 +410,12 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
     y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * RS; y.out_b1 = NY * RS;
     y.nb2 = nf; y.in_b2 = NZ * b.CY * RS; y.out_b2 = NZ * NY * RS;
     y.nch = nch; y.in_ch = a_ch; y.out_ch = b_ch;
-    const double ey = (double)nf * NZ * b.KX * 2.0 * sizeof(T);
-    LAPS_CK(timed(c, K_YINV, nch * ey * (b.CY + NY),
+    if (mask) {
+        y.mask = mask; y.row_role = 2;
+        y.mX = NX; y.mY = NY; y.mZ = NZ;
+    }
+    const double ey = (double)nf * NZ * 2.0 * sizeof(T);
+    LAPS_CK(timed(c, K_YINV, nch * ey * (rows + (double)b.KX * NY),
                   [&] { return launch_strided<T, T>(c->ny, +1, A, B, y, tw_of<T>(c, 1), T(1), c->stream, plan_y); }));
     if (!dst) return LAPS_OK;
     XArgs x{};
@@ -501,7 +514,7 @@ int substage(laps_ctx* c, int irk) {
                 return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
             }));
         if ((rc = inv_pipeline<T, T>(c, state_src<T>(c), KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv,
-                                     c->plan_yinv, nch == 2, nch, c->p1, c->p2)))
+                                     c->plan_yinv, nch == 2, nch, c->p1, c->p2, true)))
             return rc;
         xa.nch = nch;
         xa.st_ch = c->p2;
@@ -614,7 +627,7 @@ int autotune(laps_ctx* c) {
             // z+y with plan for the y part held at 0, then y alone with plan
             LAPS_CK(hipEventRecord(e0, c->stream));
             int rc = inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, plan, 0,
-                                        nch == 2, nch, c->p1, c->p2);
+                                        nch == 2, nch, c->p1, c->p2, true);
             if (rc) return rc;
             LAPS_CK(hipEventRecord(e1, c->stream));
             LAPS_CK(hipEventSynchronize(e1));
@@ -627,6 +640,10 @@ int autotune(laps_ctx* c) {
             y.nb1 = (int)NZ; y.in_b1 = (long long)k.CY * RS; y.out_b1 = NY * RS;
             y.nb2 = c->nf_in(); y.in_b2 = NZ * k.CY * RS; y.out_b2 = NZ * NY * RS;
             y.nch = nch; y.in_ch = c->p1; y.out_ch = c->p2;
+            if (c->p.dealias_option == 1 || c->p.dealias_option == 3) {  // as in the substage
+                y.mask = c->p.dealias_option; y.row_role = 2;
+                y.mX = c->nx; y.mY = NY; y.mZ = NZ;
+            }
             LAPS_CK(hipEventRecord(e0, c->stream));
             LAPS_CK(launch_strided<T, T>(c->ny, +1, P1, P2, y, tw_of<T>(c, 1), T(1), c->stream, plan));
             LAPS_CK(hipEventRecord(e1, c->stream));
