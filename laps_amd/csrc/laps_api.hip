@@ -299,6 +299,15 @@ int compute_kept_box(laps_ctx* c, std::vector<int>& kzlim) {
         const int iy = (b.CY == ny || cy <= (b.CY - 1) / 2) ? cy : ny - (b.CY - cy);
         for (int kx = 0; kx < b.KX; ++kx) kzlim[(size_t)cy * b.KX + kx] = col[(size_t)iy * nxh + kx];
     }
+    // the inverse passes drop a row by an integer test of its mz = 0 mode
+    // (mask_row_dead); it must never drop a row that holds a kept mode
+    for (int cy = 0; cy < b.CY; ++cy) {
+        const int iy = (b.CY == ny || cy <= (b.CY - 1) / 2) ? cy : ny - (b.CY - cy);
+        for (int kx = 0; kx < b.KX; ++kx)
+            if (kzlim[(size_t)cy * b.KX + kx] >= 0 &&
+                mask_row_dead(c->p.dealias_option, nx, ny, nz, kx, signed_index(iy, ny)))
+                return fail(LAPS_EINVAL, "internal: the row test of the inverse passes drops a kept mode");
+    }
     c->nkept = c->nlive = 0;
     for (int lim : kzlim) {
         if (lim < 0) continue;
