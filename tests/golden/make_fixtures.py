@@ -1,10 +1,16 @@
-"""Writes tests/golden/*.npz: small golden vectors of the LAPS step.
+"""Writes tests/golden/*.npz: small regression vectors of the LAPS step.
 
-The reference ships no tests or fixtures and cannot be built here (its FFTs
-need FFTW3, absent from this image), so these vectors come from the pinned
-CPU oracle (oracle/laps_oracle.c). They freeze its output: a change to the
-oracle or to the initial-condition generator that moves a result fails the
-CPU test, and the GPU path is checked against the same numbers.
+These are frozen output of the CPU oracle (oracle/laps_oracle.c), not an
+independent pin of it. The reference ships no tests or fixtures, and its own
+step cannot be built here without a stand-in: its transforms call FFTW3
+(fftw.f90:8, not vendored, absent from this image) and it needs MPI. Most of
+the reference's sources compile with amdflang; fftw.f90 and its dependants
+(mhdrhs.f90, mhd.f90) do not, and building them against a stand-in FFTW
+would not be the reference. What pins the oracle is tests/test_oracle_pins.py
+and tests/test_phase_shift.py (library routines, brute force, closed forms,
+invariants). These vectors only catch a change that moves a result: a change
+to the oracle or to the initial-condition generator fails the CPU test, and
+the GPU path is checked against the same numbers.
 
     python tests/golden/make_fixtures.py
 """

@@ -9,6 +9,7 @@ import math
 import numpy as np
 import pytest
 
+from laps_amd.ic import cons_from_prim, seeded_primitive
 from laps_amd.params import LAPS_DT_STOP, LAPS_MIXED
 from tests.helpers import kept_modes, params, rel_l2, seeded_state, TWO_PI
 
@@ -257,3 +258,72 @@ def test_state_beyond_the_mask(laps_mod, oracle_mod, dealias, precision):
     finally:
         g.close()
         o.close()
+
+
+def contrast_state(p):
+    """A state whose density and pressure vary by tens of per cent (the seeded states start at
+    uniform rho and p): rho = 1 + 0.3 sin x sin 2y cos z + 0.2 cos(3z + y), p = 1 + 0.3 cos(2x - z),
+    with the seeded u and B of seed 9."""
+    prim = seeded_primitive((p.nx, p.ny, p.nz), 9, kmin=1, kmax=4, du=0.2, db=0.2)
+    x, y, z = (np.arange(q) * TWO_PI / q for q in (p.nx, p.ny, p.nz))
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    prim[..., 0] = 1 + 0.3 * np.sin(X) * np.sin(2 * Y) * np.cos(Z) + 0.2 * np.cos(3 * Z + Y)
+    prim[..., 7] = 1 + 0.3 * np.cos(2 * X - Z)
+    return cons_from_prim(prim, p.adiabatic_index)
+
+
+@pytest.mark.parametrize("case", [
+    dict(if_hall=1, ion_inertial_length=0.05),
+    dict(if_hall=1, ion_inertial_length=0.05, dealias_option=3),
+    dict(if_hall=1, ion_inertial_length=0.05, if_aeb=1, if_corotating=1, corotating_angle=0.4, radius0=1.0,
+         Ur0=0.5),
+])
+def test_density_pressure_contrast_fp64(laps_mod, oracle_mod, case):
+    # the rho-weighted terms (u = rho u / rho in the momentum and energy fluxes, the Hall d_i / rho)
+    # and, with the expanding box restarted at R = 2 R0, the scaled and rotated wave numbers
+    p = params(32, **case)
+    uu, pr = contrast_state(p)
+    g = laps_mod.Laps(p)
+    o = oracle_mod.Oracle(p)
+    g.set_state(uu, pr)
+    o.set_state(uu, pr)
+    if p.if_aeb:
+        g.set_time(2.0, 0.0)
+        o.set_time(2.0, 0.0)
+    g.step(3)
+    o.step(3)
+    sg, so = g.scalars(), o.scalars()
+    assert sg["dt"] == pytest.approx(so["dt"], rel=1e-13)
+    assert sg["radius"] == pytest.approx(so["radius"], rel=1e-14)
+    err = rel_l2(g.spectral(), o.spectral())
+    print(f"case={case} rel_l2={err:.3e}")
+    assert err < FP64_TOL
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("ratio,kept", [(1.01, True), (0.99, True), (1.03, False), (0.97, False)])
+def test_dt_hysteresis_vs_closed_form(laps_mod, oracle_mod, ratio, kept):
+    # vardt (mhd.f90:422-428) on a uniform state, whose CFL dt never changes: the previous dt is
+    # kept within 2 % and replaced by the CFL dt otherwise
+    n = 16
+    p = params(n, cfl=0.4)
+    prim = np.zeros((n, n, n, 8), order="F")
+    prim[..., 0] = 1.0
+    prim[..., 4] = 1.0
+    prim[..., 7] = 1.0
+    uu, pr = cons_from_prim(prim, p.adiabatic_index)
+    g = laps_mod.Laps(p)
+    g.set_state(uu, pr)
+    raw, _ = g.cfl()
+    dx = TWO_PI / n
+    gam = p.adiabatic_index
+    assert raw == pytest.approx(min(dx / math.sqrt(gam), dx / math.sqrt(gam + 1)), rel=1e-14)
+    dtmin = 0.4 * raw
+    g.set_time(0.0, ratio * dtmin)
+    g.step(2)
+    sc = g.scalars()
+    want = ratio * dtmin if kept else dtmin
+    assert sc["dt"] == pytest.approx(want, rel=1e-15)
+    assert sc["time"] == pytest.approx(2 * want, rel=1e-14)
+    g.close()

@@ -306,3 +306,212 @@ def test_circular_wave_closed_form(oracle_mod, di, dealias):
         h = 0.5 * di * math.sqrt(36.0)
         other = rk3_amplification(-1j * 6.0 * (math.sqrt(1 + h * h) + h) * dt) ** nst
         assert abs(other - g) > 1e-2
+
+
+# ---------------------------------------------------------------------------------------------
+# Closed forms of the individual flux terms (calc_flux, mhdrhs.f90:60-119) on states where the
+# density, the pressure or the field vary, so that each term is tested on its own: every field
+# depends on x only (or y only), and the RHS at the lowest modes follows from one flux.
+
+def x_profile(n):
+    return np.arange(n) * TWO_PI / n
+
+
+def test_rhs_momentum_flux_nonuniform_density(oracle_mod):
+    # rho = 1 + eps cos x, u = (U, V, 0) uniform, p and B uniform: the x-momentum x-flux is
+    # rho U^2 + const and the y-momentum x-flux rho U V (mhdrhs.f90:67-79, m_i u_j with u = m/rho),
+    # so fnl2(1) = -i U^2 rho_hat and fnl3(1) = -i U V rho_hat, nothing at any other mode
+    n, U, V, eps = 16, 0.3, -0.2, 1e-2
+    p = params(n, if_visc=0, if_resis=0)
+    o = oracle_mod.Oracle(p)
+    x = x_profile(n)
+    prim = uniform_prim(n, u=(U, V, 0), b=(0.7, 0, 0), pr=1.3)
+    prim[..., 0] = 1 + eps * np.cos(x)[:, None, None]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    f = o.rhs()
+    rho_hat = eps / 2
+    assert abs(f[1, 0, 0, 1] - (-1j * U * U * rho_hat)) < 1e-15
+    assert abs(f[1, 0, 0, 2] - (-1j * U * V * rho_hat)) < 1e-15
+    mask = np.ones(f.shape[:3], bool)
+    mask[1, 0, 0] = False
+    for v in (1, 2, 3):
+        assert np.abs(f[..., v][mask]).max() < 1e-15
+    assert np.abs(f[1, 0, 0, 3]) < 1e-15
+
+
+def test_rhs_energy_flux_pressure_term(oracle_mod):
+    # p = p0 + eps cos x, u = (U, 0, 0), B = 0, rho = 1: the energy flux (e + ptot) u_x with
+    # e = p/(gamma-1) + U^2/2 gives fnl8(1) = -i U gamma/(gamma-1) p_hat (mhdrhs.f90:87-89)
+    n, U, eps = 16, 0.4, 1e-2
+    p = params(n, if_visc=0, if_resis=0)
+    g = p.adiabatic_index
+    o = oracle_mod.Oracle(p)
+    prim = uniform_prim(n, u=(U, 0, 0), b=(0, 0, 0))
+    prim[..., 7] = 1 + eps * np.cos(x_profile(n))[:, None, None]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    f = o.rhs()
+    assert abs(f[1, 0, 0, 7] - (-1j * U * g / (g - 1) * eps / 2)) < 1e-14
+    assert np.abs(f[2:, :, :, 7]).max() < 1e-15
+
+
+def test_rhs_energy_flux_magnetic_pressure_term(oracle_mod):
+    # B = (0, beta cos x, 0), u = (U, 0, 0), rho = p = 1: e + ptot = const + beta^2 cos^2 x
+    # (the magnetic pressure appears in e and in ptot), u.B = 0, so the energy flux is
+    # U beta^2 cos^2 x + const and fnl8(2) = -2i U beta^2 / 4; with e + p in its place it
+    # would be half that (mhdrhs.f90:87-89)
+    n, U, beta = 16, 0.4, 0.1
+    p = params(n, if_visc=0, if_resis=0)
+    o = oracle_mod.Oracle(p)
+    prim = uniform_prim(n, u=(U, 0, 0), b=(0, 0, 0))
+    prim[..., 5] = beta * np.cos(x_profile(n))[:, None, None]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    f = o.rhs()
+    assert abs(f[2, 0, 0, 7] - (-1j * U * beta * beta / 2)) < 1e-15
+    assert abs(f[1, 0, 0, 7]) < 1e-15 and np.abs(f[3:, :, :, 7]).max() < 1e-15
+
+
+def test_rhs_energy_flux_udotb_term(oracle_mod):
+    # u = (0, V, 0), B = (B0, beta cos x, 0), rho = p = 1: the x energy flux is
+    # (e + ptot) u_x - (u.B) B_x = -V beta B0 cos x, so fnl8(1) = +i V beta B0 / 2; the sign
+    # of the (u.B) B term decides the sign (mhdrhs.f90:87-89)
+    n, V, beta, B0 = 16, 0.3, 0.05, 1.2
+    p = params(n, if_visc=0, if_resis=0)
+    o = oracle_mod.Oracle(p)
+    prim = uniform_prim(n, u=(0, V, 0), b=(B0, 0, 0))
+    prim[..., 5] = beta * np.cos(x_profile(n))[:, None, None]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    f = o.rhs()
+    assert abs(f[1, 0, 0, 7] - (1j * V * beta * B0 / 2)) < 1e-15
+
+
+@pytest.mark.parametrize("rho0", [1.0, 2.0])
+def test_rhs_hall_density_factor(oracle_mod, rho0):
+    # the whistler-sign state at density rho0: the Hall EMF is (d_i/rho) J x B
+    # (mhdrhs.f90:106-119), so fnl(By) at mz = +1 is -i d_i B0 b / (2 rho0)
+    n, b, B0, di = 16, 1e-3, 1.0, 0.2
+    p = params(n, if_hall=1, ion_inertial_length=di, if_visc=0, if_resis=0)
+    o = oracle_mod.Oracle(p)
+    z = x_profile(n)
+    prim = uniform_prim(n, rho=rho0, b=(0, 0, B0))
+    prim[..., 4] = b * np.sin(z)[None, None, :]
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    f = o.rhs()
+    assert abs(f[0, 0, 1, 5] - (-1j * di * B0 * b / (2 * rho0))) < 1e-17
+    assert abs(f[0, 0, n - 1, 5] - (1j * di * B0 * b / (2 * rho0))) < 1e-17
+
+
+# ---------------------------------------------------------------------------------------------
+# Expanding box: the derivative factors k' (mhdrhs.f90:191-204, :313-326) and k'^2
+# (AEBmod.f90:87-125) at a radius R != R0, set through the restart time (evolve_radius):
+# plain AEB k' = (kx, ky R0/R, kz R0/R); corotating k' = (kx c + ky s, (ky c - kx s) R0/R,
+# kz R0/R) with (c, s) the cosine and sine of the corotating angle.
+
+R0_AEB, UR_AEB, T_AEB = 1.0, 0.5, 2.0       # R = R0 + Ur t = 2: R0/R = 1/2, tau = R/Ur = 4
+
+
+def aeb_oracle(oracle_mod, n, prim, corot=False, angle=0.0, **kw):
+    p = params(n, if_aeb=1, radius0=R0_AEB, Ur0=UR_AEB, if_corotating=1 if corot else 0,
+               corotating_angle=angle, if_visc=0, if_resis=0, **kw)
+    o = oracle_mod.Oracle(p)
+    uu, pr = oracle_mod.cons_from_prim(p, prim)
+    o.set_state(uu, pr)
+    o.set_time(T_AEB, 0.0)
+    assert o.scalars()["radius"] == pytest.approx(2.0, rel=1e-15)
+    return o
+
+
+def kprime(kx, ky, kz, corot, angle, r0r):
+    if corot:
+        c, s = math.cos(angle), math.sin(angle)
+        return kx * c + ky * s, (ky * c - kx * s) * r0r, kz * r0r
+    return kx, ky * r0r, kz * r0r
+
+
+def test_aeb_density_mode_along_y(oracle_mod):
+    # rho = 1 + eps cos y, u = (0, V, 0): fnl1(0,1,0) = -i (R0/R) V rho_hat - 2 rho_hat / tau
+    n, V, eps = 16, 0.3, 1e-2
+    prim = uniform_prim(n, u=(0, V, 0))
+    prim[..., 0] = 1 + eps * np.cos(x_profile(n))[None, :, None]
+    o = aeb_oracle(oracle_mod, n, prim)
+    f = o.rhs()
+    rho_hat, r0r, tau = eps / 2, R0_AEB / 2.0, 2.0 / UR_AEB
+    assert abs(f[0, 1, 0, 0] - (-1j * r0r * V * rho_hat - 2 * rho_hat / tau)) < 1e-15
+    assert abs(f[0, n - 1, 0, 0] - (1j * r0r * V * rho_hat - 2 * rho_hat / tau)) < 1e-15
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_aeb_corotating_mode_mixing(oracle_mod, axis):
+    # rho = 1 + eps cos(x) (axis 0) or cos(y) (axis 1), u = (U, V, 0): the mass RHS at the mode
+    # is -i (kx' U + ky' V) rho_hat - 2 rho_hat / tau with the rotated, scaled k' of that mode
+    n, U, V, eps, ang = 16, 0.3, -0.25, 1e-2, 0.4
+    prim = uniform_prim(n, u=(U, V, 0))
+    prof = np.cos(x_profile(n))
+    prim[..., 0] = 1 + eps * (prof[:, None, None] if axis == 0 else prof[None, :, None])
+    o = aeb_oracle(oracle_mod, n, prim, corot=True, angle=ang)
+    f = o.rhs()
+    m = (1, 0, 0) if axis == 0 else (0, 1, 0)
+    kx, ky, _ = kprime(*m, True, ang, R0_AEB / 2.0)
+    rho_hat, tau = eps / 2, 2.0 / UR_AEB
+    want = -1j * (kx * U + ky * V) * rho_hat - 2 * rho_hat / tau
+    assert abs(f[m[0], m[1], m[2], 0] - want) < 1e-15
+    # the same mode with the opposite sign of the rotation's kx s term differs by O(eps)
+    kx2, ky2 = (kx, -ky) if axis == 0 else (kx, ky)
+    if axis == 0:
+        assert abs(-1j * (kx2 * U + ky2 * V) * rho_hat - 2 * rho_hat / tau - want) > 1e-4
+
+
+def test_aeb_hall_whistler_scaling(oracle_mod):
+    # B = (b sin z, 0, B0), u = 0, rho = 2 at R = 2 R0: J_y = (R0/R) b cos z (the current's
+    # kz', mhdrhs.f90:313-326) and dBy/dt = -(R0/R) d_z E_x (the curl's kz', :198-204):
+    # fnl(By)(mz=1) = -i (R0/R)^2 (d_i/rho) b B0 / 2; Bx decays at 2/tau (AEB, :235-251)
+    n, b, B0, di, rho0 = 16, 1e-3, 1.0, 0.2, 2.0
+    prim = uniform_prim(n, rho=rho0, b=(0, 0, B0))
+    prim[..., 4] = b * np.sin(x_profile(n))[None, None, :]
+    o = aeb_oracle(oracle_mod, n, prim, if_hall=1, ion_inertial_length=di)
+    f = o.rhs()
+    r0r, tau = R0_AEB / 2.0, 2.0 / UR_AEB
+    assert abs(f[0, 0, 1, 5] - (-1j * r0r * r0r * di * B0 * b / (2 * rho0))) < 1e-17
+    bx_hat = b / 2j
+    assert abs(f[0, 0, 1, 4] - (-2 * bx_hat / tau)) < 1e-17
+
+
+@pytest.mark.parametrize("corot", [False, True])
+def test_aeb_ksquare_is_norm_of_kprime(oracle_mod, corot):
+    # update_ksquare (AEBmod.f90:87-125) expands |k'|^2; compare with |k'|^2 formed from the
+    # rotation and the scaling of the derivative factors
+    n, ang = 16, 0.4
+    o = aeb_oracle(oracle_mod, n, uniform_prim(n), corot=corot, angle=ang)
+    got = o.ksquare()
+    kx = np.arange(n // 2 + 1)[:, None, None].astype(float)
+    m = np.arange(n)
+    m = np.where(m <= n // 2, m, m - n).astype(float)   # +Nyquist (mhdinit.f90:83-111)
+    ky, kz = m[None, :, None], m[None, None, :]
+    a, b_, c_ = kprime(kx, ky, kz, corot, ang, R0_AEB / 2.0)
+    want = a * a + b_ * b_ + c_ * c_
+    assert np.abs(got - want).max() < 1e-12 * want.max()
+
+
+# ---------------------------------------------------------------------------------------------
+# vardt's hysteresis (mhd.f90:422-428): the previous dt is kept while it is within 2 % of the
+# new CFL dt and replaced by it otherwise. On a uniform state the CFL dt never changes.
+
+@pytest.mark.parametrize("ratio,kept", [(1.01, True), (0.99, True), (1.03, False), (0.97, False)])
+def test_dt_hysteresis(oracle_mod, ratio, kept):
+    n = 16
+    p = params(n, cfl=0.4)
+    o = oracle_mod.Oracle(p)
+    uu, pr = oracle_mod.cons_from_prim(p, uniform_prim(n))
+    o.set_state(uu, pr)
+    raw, _ = o.cfl()
+    dtmin = 0.4 * raw
+    dt_prev = ratio * dtmin
+    o.set_time(0.0, dt_prev)
+    o.step(2)
+    want = dt_prev if kept else dtmin
+    assert o.scalars()["dt"] == want
+    assert o.scalars()["time"] == pytest.approx(2 * want, rel=1e-15)
