@@ -78,7 +78,7 @@ void laps_default_params(laps_params* p);
  * fftw_initialize, grid_initialize, arrays_initialize, AEB_initialize and
  * dealias_initialize (mhd.f90:58-99; parallel.f90:28-212; fftw.f90:21-39;
  * mhdinit.f90:58-181; AEBmod.f90:16-44; dealiasing.f90:13-67).
- * Supported lengths per axis: 16, 32, 64, 128, 256, 512. */
+ * Supported lengths per axis: 16, 32, 64, 96, 128, 256, 512. */
 int laps_create(const laps_params* p, laps_ctx** out);
 
 /* Frees every device allocation of the context (fftw_finalize, fftw.f90:225). */
