@@ -260,8 +260,7 @@ contains
             enddo
             uu(:, :, :, 8) = press0
             uu(:, :, :, 5) = bx0
-            uu(:, :, :, 6) = by0
-            uu(:, :, :, 7) = bz0
+            uu(:, :, :, 6) = by0   ! Bz is left as it is (mhdinit.f90:245-249)
         case (3)
             uu(:, :, :, 5) = bx0
             uu(:, :, :, 6) = by0
@@ -294,7 +293,7 @@ contains
         end select
     end subroutine background_fields_initialize
 
-    ! perturbation_initialize (mhdinit.f90:302-1036): ipert = 1 and 999. The
+    ! perturbation_initialize (mhdinit.f90:302-1036): ipert = 1, 8 and 999. The
     ! other hooks draw from the compiler's RNG; start those runs from an
     ! out000.dat written by laps_amd.ic and if_restart = .true.
     subroutine perturbation_initialize
@@ -315,6 +314,8 @@ contains
                 uu(ix, :, :, 3) = uu(ix, :, :, 3) + db0 / sqrt(uu(ix, :, :, 1)) * cos(kx * xgrid(ix)) * c
                 uu(ix, :, :, 6) = uu(ix, :, :, 6) - db0 * cos(kx * xgrid(ix)) * c
             enddo
+        case (8)
+            call switchback
         case (999)
             block
                 integer :: iy, iz, v
@@ -332,6 +333,71 @@ contains
             error stop 3
         end select
     end subroutine perturbation_initialize
+
+    ! ipert = 8 (mhdinit.f90:931-1020): a switchback, a localised reversal of the field along x
+    ! centred in the box, added to the background (which should carry no field of its own).
+    ! The x component (the reference's "Bz" of the structure) is B0 + dB g(r) h(x) with the
+    ! Gaussian ring g(r) = exp(-((r - R0)/R1)^2) in the y-z radius r and h(x) = exp(-(x/H)^2);
+    ! the radial component Br follows from div B = 0, and the azimuthal one fills |B| up to
+    ! B0 where it can (elsewhere the pressure is lowered to keep p + B^2/2 uniform). Beyond
+    ! r = Rm both are tapered with exp(-((r - Rm)/Rd)^2), the x component corrected with q so
+    ! that the field stays solenoidal. The velocity is then anti-correlated, u -= B/sqrt(rho).
+    subroutine switchback
+        integer :: ix, iy, iz
+        real(dp) :: xs, ys, zs, r, gring, gx, bx_core, bx, br, bphi, cconst, phi, taper, dtaper, q, sflux
+        real(dp) :: b_y, b_z
+        cconst = -(exp(-(R0_SB / R1_SB)**2) + sqrt(pi) * R0_SB / R1_SB * erf(R0_SB / R1_SB))
+        do iz = 1, nz
+            zs = zgrid(iz) - Lz / 2
+            do iy = 1, ny
+                ys = ygrid(iy) - Ly / 2
+                r = sqrt(ys**2 + zs**2)
+                gring = exp(-((r - R0_SB) / R1_SB)**2)
+                do ix = 1, nx
+                    xs = xgrid(ix) - Lx / 2
+                    gx = exp(-(xs / H_SB)**2)
+                    bx_core = B0_SB + dB_SB * gring
+                    bx = B0_SB + dB_SB * exp(-((r - R0_SB) / R1_SB)**2 - (xs / H_SB)**2)
+                    if (r < 1e-8_dp) then
+                        phi = 0
+                        br = 0
+                        bphi = 0
+                    else
+                        phi = atan2(zs, ys)
+                        ! the radial flux function of the ring, r Br = -dB R1^2 x/H^2 gx sflux
+                        sflux = gring + sqrt(pi) * R0_SB / R1_SB * erf((R0_SB - r) / R1_SB) + cconst
+                        br = -dB_SB * (xs * R1_SB**2) / (r * H_SB**2) * gx * sflux
+                        if (r < Rm_SB) then
+                            taper = 1
+                            dtaper = 0
+                            q = 1
+                        else
+                            taper = exp(-((r - Rm_SB) / Rd_SB)**2)
+                            dtaper = -2 * (r - Rm_SB) / Rd_SB**2 * taper
+                            q = bx_core / bx * (1 + 1 / bx_core * (taper * (bx - bx_core) - &
+                                (dtaper * dB_SB * R1_SB**2 / r * sflux * 0.5_dp * (gx - 1))))
+                        endif
+                        br = br * taper
+                        bx = bx * q
+                        if (bx**2 + br**2 < B0_SB**2) then
+                            bphi = sqrt(B0_SB**2 - br**2 - bx**2)
+                        else
+                            bphi = 0
+                            uu(ix, iy, iz, 8) = uu(ix, iy, iz, 8) - 0.5_dp * (bx**2 + br**2 - B0_SB**2)
+                        endif
+                    endif
+                    b_y = br * cos(phi) - bphi * sin(phi)
+                    b_z = br * sin(phi) + bphi * cos(phi)
+                    uu(ix, iy, iz, 5) = uu(ix, iy, iz, 5) + bx
+                    uu(ix, iy, iz, 6) = uu(ix, iy, iz, 6) + b_y
+                    uu(ix, iy, iz, 7) = uu(ix, iy, iz, 7) + b_z
+                enddo
+            enddo
+        enddo
+        uu(:, :, :, 2) = uu(:, :, :, 2) - uu(:, :, :, 5) / sqrt(uu(:, :, :, 1))
+        uu(:, :, :, 3) = uu(:, :, :, 3) - uu(:, :, :, 6) / sqrt(uu(:, :, :, 1))
+        uu(:, :, :, 4) = uu(:, :, :, 4) - uu(:, :, :, 7) / sqrt(uu(:, :, :, 1))
+    end subroutine switchback
 
     ! initial_calc_conserve_variable (mhdinit.f90:1038-1056)
     subroutine initial_calc_conserve_variable
