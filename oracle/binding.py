@@ -13,7 +13,9 @@ import numpy as np
 from laps_amd.params import LapsParams
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liblaps_oracle.so")
+# LAPS_ORACLE_LIB names another build of the same source (tests/test_oracle_mutations.py
+# loads scratch builds with planted mistakes through it)
+_LIB_PATH = os.environ.get("LAPS_ORACLE_LIB") or os.path.join(_HERE, "liblaps_oracle.so")
 _lib = None
 
 _D = ctypes.POINTER(ctypes.c_double)
