@@ -170,6 +170,7 @@ def run_ensemble(steps):
         if graph:
             ens = Ensemble([p for p, _, _ in spec])
             boxes = ens.boxes
+            ngroups = ens.groups()
         else:
             boxes = [Laps(p) for p, _, _ in spec]
         for b, (uu, pr) in zip(boxes, states):
@@ -211,13 +212,62 @@ def run_ensemble(steps):
     nb = len(spec)
     return dict(workload="96 independent boxes: 32 x 32^3, 32 x 64^3, 32 x 96^3 (seeds 100-195, db/B0 = du "
                          "uniform in [0.1, 0.5], d_i 0 / 0.05), mixed precision, 2/3 rule, CFL 0.4, own CFL dt per box",
-                steps=steps, box_steps_per_s=round(nb * steps / g["seconds"], 1),
+                steps=steps, launch_groups=ngroups, box_steps_per_s=round(nb * steps / g["seconds"], 1),
                 box_steps_per_s_per_box_streams=round(nb * steps / e["seconds"], 1),
                 graph_speedup=round(e["seconds"] / g["seconds"], 3),
                 grid_point_steps_per_s=round(pts * steps / g["seconds"], 1),
                 ms_per_ensemble_step=round(1000.0 * g["seconds"] / steps, 4),
                 graph_equals_per_box_steps=bool(same),
                 time_range=[min(x["time"] for x in g["scalars"]), max(x["time"] for x in g["scalars"])])
+
+
+def run_output_staging(n, uu, pr, dealias, steps=40, every=10):
+    """Steps/s of the FP64 workload with an output every `every` steps, the payload of
+    outNNN.dat read (a) with laps_get_state (the host waits for the inverse transforms and a
+    pageable download) and (b) staged on the device (laps_stage_output: the copy into pinned
+    memory runs while the next step does; laps_staged_output then finds it done), against the
+    rate without outputs. The payload is written to /dev/null in both cases."""
+    from laps_amd.params import LAPS_FP64
+    from laps_amd.runtime import Laps
+    out = {}
+    with Laps(workload_params(n, LAPS_FP64, dealias)) as g, open(os.devnull, "wb") as sink:
+        g.set_state(uu, pr)
+        g.step(2)
+        g.sync()
+        for mode in ("none", "get_state", "staged"):
+            wait = 0.0
+            pending = False
+            t0 = time.perf_counter()
+            for i in range(steps):
+                g.step(1)
+                if pending:
+                    t1 = time.perf_counter()
+                    w = g.staged_output(copy=False)[0]
+                    wait += time.perf_counter() - t1
+                    w.T.tofile(sink)
+                    pending = False
+                g.scalars()   # the host loop reads time and dt every step (mhd.f90:216-285)
+                if (i + 1) % every:
+                    continue
+                if mode == "get_state":
+                    t1 = time.perf_counter()
+                    u, q = g.state()
+                    u[..., 1:4] = q[..., 0:3]
+                    u[..., 7] = q[..., 3]
+                    wait += time.perf_counter() - t1
+                    u.T.tofile(sink)
+                elif mode == "staged":
+                    g.stage_output(True)
+                    pending = True
+            if pending:
+                g.staged_output(copy=False)[0].T.tofile(sink)
+            g.sync()
+            sec = time.perf_counter() - t0
+            out[mode] = dict(steps_per_s=round(steps / sec, 3), host_wait_s_per_output=round(wait / (steps // every), 4)
+                             if mode != "none" else 0.0)
+            log(f"output staging {mode}: {steps} steps in {sec:.3f} s")
+    return dict(workload=f"the FP64 {n}^3 workload, {steps} steps with an outNNN.dat payload "
+                         f"({8 * n ** 3 * 8 / 1e9:.2f} GB) every {every} steps", **out)
 
 
 def config2_params(precision):
@@ -236,29 +286,12 @@ def config2_state(seed=2):
     """Outward Alfven-wave spectrum, amplitude k^-1 in shells 1-32, db/B0 = 0.5, du = -db/sqrt(rho),
     rho = p = 1, B0 = (1, 0, 0). The field has no mode beyond shell 32, so it is generated on 128^3 (same
     seeded modes, same rms) and brought to 512^3 by the library's own inverse transform of its spectrum."""
-    from laps_amd.ic import cons_from_prim, seeded_primitive
+    from laps_amd.ic import band_limited_state
     from laps_amd.params import LAPS_FP64
     from laps_amd.runtime import Laps
-    m, n = 128, 512
-    pc = seeded_primitive((m, m, m), seed, kmin=1, kmax=32, slope=-1.0, du=0.5, db=0.5, alfvenic=True)
-    idx = np.arange(m)
-    idx = np.where(idx <= m // 2, idx, idx - m) % n
-    spec = np.zeros((n // 2 + 1, n, n, 6), dtype=np.complex128, order="F")
-    for f, (col, off) in enumerate(((1, 0.0), (2, 0.0), (3, 0.0), (4, 1.0), (5, 0.0), (6, 0.0))):
-        F = np.fft.rfftn(pc[..., col] - off, axes=(2, 1, 0)) / m ** 3
-        spec[np.ix_(np.arange(m // 2 + 1), idx, idx, [f])] = F[..., None]
-    del pc
     with Laps(config2_params(LAPS_FP64)) as g:
-        real = g.fft_c2r(spec)
-    del spec
-    prim = np.zeros((n, n, n, 8), order="F")
-    prim[..., 0] = 1.0
-    prim[..., 7] = 1.0
-    prim[..., 1:4] = real[..., 0:3]
-    prim[..., 4:7] = real[..., 3:6]
-    prim[..., 4] += 1.0
-    del real
-    return cons_from_prim(prim, 5.0 / 3.0)
+        return band_limited_state(512, 128, 5.0 / 3.0, g.fft_c2r, seed=seed, kmin=1, kmax=32, slope=-1.0,
+                                  du=0.5, db=0.5, alfvenic=True)
 
 
 def run_config2(steps):
@@ -367,6 +400,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="steps of the 96-box ensemble reported under 'ensemble' (0: skip it)")
+    ap.add_argument("--output-every", type=int, default=10,
+                    help="steps between outputs in the output-staging comparison (0: skip it)")
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the learned-closure experiment reported under 'closure'")
     ap.add_argument("--config2", type=int, default=0, metavar="STEPS",
@@ -478,8 +513,16 @@ def main():
         for v in ro.values():
             v.pop("spectral", None)
         out["dealias_2_3" if other == 1 else "dealias_phase_shift"] = rep
+    if args.output_every > 0:
+        out["output_staging"] = run_output_staging(n, uu, pr, dl, every=args.output_every)
     if args.ensemble_steps > 0:
         out["ensemble"] = run_ensemble(args.ensemble_steps)
+        m23 = out.get("dealias_2_3", {}).get("mixed") if dl == 3 else out.get("mixed")
+        if m23:
+            # the same precision and dealiasing on one 256^3 box: grid-point steps per second
+            single = m23["value"] * float(n) ** 3
+            out["ensemble"]["single_box_grid_point_steps_per_s"] = round(single, 1)
+            out["ensemble"]["vs_single_box"] = round(out["ensemble"]["grid_point_steps_per_s"] / single, 3)
     if not args.no_closure:
         from laps_amd.closure import experiment
         out["closure"] = experiment(log=log)

@@ -90,6 +90,20 @@ module laps_c
             real(c_double), intent(out) :: uu(*), uu_prim(*)
         end function
 
+        integer(c_int) function laps_stage_output(ctx, primitive) bind(C, name='laps_stage_output')
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: primitive
+        end function
+
+        integer(c_int) function laps_staged_output(ctx, payload, time, dt, istep) bind(C, name='laps_staged_output')
+            import :: c_ptr, c_int, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            type(c_ptr), intent(out) :: payload
+            real(c_double), intent(out) :: time, dt
+            integer(c_int64_t), intent(out) :: istep
+        end function
+
         integer(c_int) function laps_moments(ctx, ave, var, rho_u2) bind(C, name='laps_moments')
             import :: c_ptr, c_int, c_double
             type(c_ptr), value :: ctx

@@ -78,6 +78,8 @@ program mhd_host
     integer(c_int64_t) :: istep
     integer(c_int32_t) :: status
     integer :: iout, ios, i
+    integer :: pending_out = -1          ! output staged on the device, written after the next step is queued
+    real(dp) :: pending_t
     integer(8) :: clock_start, clock_now, clock_rate, clock_max
     integer(8), parameter :: dstep_checksave = 40
 
@@ -167,7 +169,11 @@ program mhd_host
     principal: do
         if (time >= tout) then
             write(*, '(2x,a,2x,f10.4,2x,a,2x,1pe12.4)') 'OUTPUT at time:', time, ',dt = ', dt
-            call output_uu(iout, time)
+            ! staged: the state is copied to pinned memory while the next step runs, and the
+            ! file is written from there once that step is queued
+            call laps_check(laps_stage_output(ctx, merge(1_c_int, 0_c_int, output_primitive)), 'laps_stage_output')
+            pending_out = iout
+            pending_t = time
             iout = iout + 1
             tout = tout + dtout
         endif
@@ -188,6 +194,10 @@ program mhd_host
 
         ! dt < 1e-8 test, evolve, time += dt, evolve_radius, next vardt (on the device)
         call laps_check(laps_step(ctx, 1), 'laps_step')
+        if (pending_out >= 0) then
+            call write_staged(pending_out, pending_t)
+            pending_out = -1
+        endif
         call laps_check(laps_get_scalars(ctx, time, dt, radius, istep, status), 'laps_get_scalars')
         if (status == LAPS_DT_STOP) then
             write(*, *) '***********************************'
@@ -314,6 +324,14 @@ contains
                 uu(ix, :, :, 3) = uu(ix, :, :, 3) + db0 / sqrt(uu(ix, :, :, 1)) * cos(kx * xgrid(ix)) * c
                 uu(ix, :, :, 6) = uu(ix, :, :, 6) - db0 * cos(kx * xgrid(ix)) * c
             enddo
+        case (2:4)
+            call random_modes_1d(ipert)
+        case (5)
+            call tearing_modes
+        case (6)
+            call alfvenic_modes
+        case (7)
+            call isotropic_modes
         case (8)
             call switchback
         case (999)
@@ -328,11 +346,236 @@ contains
                 enddo
             end block
         case default
-            write(*, '(1x,a,i0,a)') 'ipert = ', ipert, ' is not built into this host: write out000.dat with ', &
-                'laps_amd.ic and restart from it (if_restart = .true., n_start = 0)'
+            write(*, '(1x,a,i0,a)') 'ipert = ', ipert, ' is not a perturbation of the reference: write out000.dat ', &
+                'with laps_amd.ic and restart from it (if_restart = .true., n_start = 0)'
             error stop 3
         end select
     end subroutine perturbation_initialize
+
+    ! The random-phase hooks below draw their phases from the compiler's random_number, in the
+    ! order and with the seeding calls of the reference (mhdinit.f90:344-829), so a run matches
+    ! the reference built with the same compiler. Each adds plane waves a cos(k.x + ph) (or sin),
+    ! evaluated as the real part of e^{i kx x} e^{i ky y} e^{i kz z} e^{i ph}.
+
+    ! w(:,:,:) += amp * Re(e^{i(k.x + ph)} * rot) for integer wave numbers (mx, my, mz) of the
+    ! box; rot = 1 gives cos, rot = -i gives sin
+    subroutine add_wave(w, mx, my, mz, amp, ph, rot)
+        real(dp), intent(inout) :: w(:, :, :)
+        integer, intent(in) :: mx, my, mz
+        real(dp), intent(in) :: amp, ph
+        complex(dp), intent(in) :: rot
+        complex(dp) :: ex(nx), ey(ny), ez(nz), cyz
+        integer :: ix, iy, iz
+        do ix = 1, nx
+            ex(ix) = exp(cmplx(0.0_dp, 2 * pi * mx / Lx * xgrid(ix), dp))
+        enddo
+        do iy = 1, ny
+            ey(iy) = exp(cmplx(0.0_dp, 2 * pi * my / Ly * ygrid(iy), dp))
+        enddo
+        do iz = 1, nz
+            ez(iz) = exp(cmplx(0.0_dp, 2 * pi * mz / Lz * zgrid(iz) + ph, dp)) * rot * amp
+        enddo
+        do iz = 1, nz
+            do iy = 1, ny
+                cyz = ey(iy) * ez(iz)
+                w(:, iy, iz) = w(:, iy, iz) + real(ex(:) * cyz, dp)
+            enddo
+        enddo
+    end subroutine add_wave
+
+    ! ipert = 2, 3, 4 (mhdinit.f90:344-392): nmode waves of amplitude db0 and random phase along
+    ! x in Bz, along y in Bz, along z in Bx, as sin(k s + ph)
+    subroutine random_modes_1d(kind)
+        integer, intent(in) :: kind
+        real(dp), allocatable :: phs(:)
+        integer :: ik
+        allocate(phs(nmode))
+        call random_number(phs)
+        phs = phs * 2 * pi
+        do ik = 1, nmode
+            select case (kind)
+            case (2)
+                call add_wave(uu(:, :, :, 7), ik, 0, 0, db0, phs(ik), (0.0_dp, -1.0_dp))
+            case (3)
+                call add_wave(uu(:, :, :, 7), 0, ik, 0, db0, phs(ik), (0.0_dp, -1.0_dp))
+            case (4)
+                call add_wave(uu(:, :, :, 5), 0, 0, ik, db0, phs(ik), (0.0_dp, -1.0_dp))
+            end select
+        enddo
+    end subroutine random_modes_1d
+
+    ! ipert = 5 (mhdinit.f90:394-483): 3D tearing perturbations of two current sheets at y = Ly/4
+    ! and 3Ly/4, Gaussian profiles of width a0: for each (kx, kz), ikx = 1..nmodex, ikz =
+    ! 0..nmodez, bx, bz += (dbx, dbz) cos(ph) m'(y) and by += (kx dbx + kz dbz) sin(ph) m(y), m =
+    ! a0 exp(-((y - yc)/a0)^2) (divergence free), each sheet on its own half of the box. The
+    ! phase of (ikx, ikz) is entry (ikx - 1) nmodez + ikz of each sheet's draw of nmodex (nmodez + 1)
+    ! numbers. Entry 0 (mode (1, 0)) lies before the reference's array; here it is drawn after
+    ! both sheets' draws.
+    subroutine tearing_modes
+        real(dp), allocatable :: phs(:, :), my(:), dmy(:)
+        real(dp) :: kx, kz, yc
+        integer :: sheet, ikx, ikz, iy, nsz, nph, idx
+        nph = nmodex * (nmodez + 1)
+        allocate(phs(0:nph, 2), my(ny), dmy(ny))
+        ! the two sheets' draws in the reference's order, then the entries it reads before them
+        do sheet = 1, 2
+            call random_seed(size=nsz)   ! the reference's call random_seed(ir) queries the size
+            call random_number(phs(1:nph, sheet))
+        enddo
+        call random_number(phs(0, :))
+        phs = phs * 2 * pi
+        do sheet = 1, 2
+            yc = merge(0.25_dp, 0.75_dp, sheet == 1) * Ly
+            do iy = 1, ny
+                if ((sheet == 1) .eqv. (ygrid(iy) <= 0.5_dp * Ly)) then
+                    my(iy) = a0 * exp(-((ygrid(iy) - yc) / a0)**2)
+                    dmy(iy) = -2 * (ygrid(iy) - yc) / a0 * exp(-((ygrid(iy) - yc) / a0)**2)
+                else
+                    my(iy) = 0
+                    dmy(iy) = 0
+                endif
+            enddo
+            do ikx = 1, nmodex
+                kx = ikx * 2 * pi / Lx
+                do ikz = 0, nmodez
+                    kz = ikz * 2 * pi / Lz
+                    idx = (ikx - 1) * nmodez + ikz
+                    call add_profiled(5, ikx, ikz, dbx, phs(idx, sheet), dmy, (1.0_dp, 0.0_dp))
+                    call add_profiled(7, ikx, ikz, dbz, phs(idx, sheet), dmy, (1.0_dp, 0.0_dp))
+                    call add_profiled(6, ikx, ikz, kx * dbx + kz * dbz, phs(idx, sheet), my, (0.0_dp, -1.0_dp))
+                enddo
+            enddo
+        enddo
+    end subroutine tearing_modes
+
+    ! uu(:,:,:,v) += amp * prof(y) * Re(e^{i(kx x + kz z + ph)} rot)
+    subroutine add_profiled(v, ikx, ikz, amp, ph, prof, rot)
+        integer, intent(in) :: v, ikx, ikz
+        real(dp), intent(in) :: amp, ph, prof(:)
+        complex(dp), intent(in) :: rot
+        complex(dp) :: ex(nx), ez
+        integer :: ix, iy, iz
+        do ix = 1, nx
+            ex(ix) = exp(cmplx(0.0_dp, 2 * pi * ikx / Lx * xgrid(ix), dp))
+        enddo
+        do iz = 1, nz
+            ez = exp(cmplx(0.0_dp, 2 * pi * ikz / Lz * zgrid(iz) + ph, dp)) * rot * amp
+            do iy = 1, ny
+                uu(:, iy, iz, v) = uu(:, iy, iz, v) + prof(iy) * real(ex(:) * ez, dp)
+            enddo
+        enddo
+    end subroutine add_profiled
+
+    ! ipert = 6 (mhdinit.f90:486-692): Alfven waves propagating along +x (outward) and -x
+    ! (inward, amplitude times in_out_ratio), modes ikx = 1..nmodex, iky, ikz in -nmode..nmode,
+    ! amplitude db0 / sqrt(|ikx| |iky| |ikz|) (a zero index counts as 1). (ky, kz) = 0: a
+    ! circularly polarised wave, by = cos, bz = sin; otherwise b along (kz, -ky)/|(ky, kz)|, cos.
+    ! u = -/+ sign(bx0) b for the outward / inward waves. One draw of phases per direction.
+    subroutine alfvenic_modes
+        real(dp), allocatable :: phs(:)
+        real(dp) :: ky, kz, amp, dby, dbz, sgn, s_uv
+        integer :: dir, ikx, iky, ikz, nsz, idx, nph
+        nph = nmodex * (2 * nmodez + 1) * (2 * nmodey + 1)
+        allocate(phs(nph))
+        sgn = sign(1.0_dp, bx0)
+        do dir = 1, 2
+            call random_seed(size=nsz)
+            call random_number(phs)
+            phs = phs * 2 * pi
+            s_uv = merge(-sgn, sgn, dir == 1)
+            do ikx = 1, nmodex
+                do iky = -nmodey, nmodey
+                    ky = iky * 2 * pi / Ly
+                    do ikz = -nmodez, nmodez
+                        kz = ikz * 2 * pi / Lz
+                        amp = db0 * sqrt(1 / real(ikx, dp)) * merge(1.0_dp, sqrt(1 / real(abs(iky), dp)), iky == 0) &
+                            * merge(1.0_dp, sqrt(1 / real(abs(ikz), dp)), ikz == 0)
+                        if (dir == 2) amp = amp * in_out_ratio
+                        idx = 1 + ((ikx - 1) * (2 * nmodey + 1) + iky + nmodey) * (2 * nmodez + 1) + ikz + nmodez
+                        if (iky == 0 .and. ikz == 0) then
+                            call add_wave4(merge(ikx, -ikx, dir == 1), 0, 0, phs(idx), amp, amp, &
+                                           (0.0_dp, -1.0_dp), s_uv)
+                        else
+                            dby = kz / sqrt(ky**2 + kz**2)
+                            dbz = -ky / sqrt(ky**2 + kz**2)
+                            call add_wave4(merge(ikx, -ikx, dir == 1), iky, ikz, phs(idx), amp * dby, amp * dbz, &
+                                           (1.0_dp, 0.0_dp), s_uv)
+                        endif
+                    enddo
+                enddo
+            enddo
+        enddo
+    end subroutine alfvenic_modes
+
+    ! one mode of ipert = 6: by += ay cos(ph'), bz += az Re(rot e^{i ph'}), and s times the same
+    ! in uy, uz (ph' = k.x + ph)
+    subroutine add_wave4(mx, my, mz, ph, ay, az, rot, s)
+        integer, intent(in) :: mx, my, mz
+        real(dp), intent(in) :: ph, ay, az, s
+        complex(dp), intent(in) :: rot
+        call add_wave(uu(:, :, :, 6), mx, my, mz, ay, ph, (1.0_dp, 0.0_dp))
+        call add_wave(uu(:, :, :, 7), mx, my, mz, az, ph, rot)
+        call add_wave(uu(:, :, :, 3), mx, my, mz, s * ay, ph, (1.0_dp, 0.0_dp))
+        call add_wave(uu(:, :, :, 4), mx, my, mz, s * az, ph, rot)
+    end subroutine add_wave4
+
+    ! ipert = 7 (mhdinit.f90:695-829): isotropic random modes in the half space ikx >= 0 (ikx = 0:
+    ! iky > 0, or iky = 0 and ikz > 0) with |k| <= max(nmodex, nmodey, nmodez), amplitude
+    ! |k|^(-3/2), along k x B0 / (|k| |B0|): u (dv0, phase ph0, and the part correlated with b with
+    ! coefficient -0.05 and phase ph1), b (db0, phase ph1), rho (drho0, phase ph2). The three
+    ! phase draws follow random_seed(put = ...) with every seed word 101, 116 and 132.
+    subroutine isotropic_modes
+        real(dp), allocatable :: ph0(:), ph1(:), ph2(:)
+        integer, allocatable :: seed(:)
+        real(dp), parameter :: corr = -0.05_dp
+        real(dp) :: kx, ky, kz, kmod, b0mod, kr, dperp(3), a
+        integer :: ikx, iky, ikz, idx, nph, nsz, c
+        nph = (2 * nmodex + 1) * (2 * nmodez + 1) * (2 * nmodey + 1)
+        allocate(ph0(nph), ph1(nph), ph2(nph))
+        call random_seed(size=nsz)
+        allocate(seed(nsz))
+        seed = 101
+        call random_seed(put=seed)
+        call random_number(ph0)
+        seed = 116
+        call random_seed(put=seed)
+        call random_number(ph1)
+        seed = 132
+        call random_seed(put=seed)
+        call random_number(ph2)
+        ph0 = ph0 * 2 * pi
+        ph1 = ph1 * 2 * pi
+        ph2 = ph2 * 2 * pi
+        b0mod = sqrt(bx0**2 + by0**2 + bz0**2)
+        do ikx = 0, nmodex
+            kx = ikx * 2 * pi / Lx
+            do iky = -nmodey, nmodey
+                ky = iky * 2 * pi / Ly
+                do ikz = -nmodez, nmodez
+                    kz = ikz * 2 * pi / Lz
+                    if ((ikx == 0 .and. iky < 0) .or. (ikx == 0 .and. iky == 0 .and. ikz <= 0)) cycle
+                    kr = sqrt(real(ikx**2 + iky**2 + ikz**2, dp))
+                    if (kr > max(nmodex, nmodey, nmodez)) cycle
+                    kmod = sqrt(kx**2 + ky**2 + kz**2)
+                    if (kmod == 0 .or. b0mod == 0) then
+                        write(*, *) 'When intializing perturbation, |k|=0 or |B0|=0!!!'
+                        error stop 0
+                    endif
+                    dperp = [ky * bz0 - kz * by0, kz * bx0 - kx * bz0, kx * by0 - ky * bx0] / kmod / b0mod
+                    idx = 1 + ((ikx + nmodex) * (2 * nmodey + 1) + iky + nmodey) * (2 * nmodez + 1) + ikz + nmodez
+                    a = 1 / sqrt(kr**3)
+                    do c = 1, 3
+                        call add_wave(uu(:, :, :, 1 + c), ikx, iky, ikz, sqrt(1 - corr**2) * dv0 * a * dperp(c), &
+                                      ph0(idx), (1.0_dp, 0.0_dp))
+                        call add_wave(uu(:, :, :, 4 + c), ikx, iky, ikz, db0 * a * dperp(c), ph1(idx), (1.0_dp, 0.0_dp))
+                        call add_wave(uu(:, :, :, 1 + c), ikx, iky, ikz, corr * dv0 * a * dperp(c), ph1(idx), &
+                                      (1.0_dp, 0.0_dp))
+                    enddo
+                    call add_wave(uu(:, :, :, 1), ikx, iky, ikz, drho0 * a, ph2(idx), (1.0_dp, 0.0_dp))
+                enddo
+            enddo
+        enddo
+    end subroutine isotropic_modes
 
     ! ipert = 8 (mhdinit.f90:931-1020): a switchback, a localised reversal of the field along x
     ! centred in the box, added to the background (which should carry no field of its own).
@@ -419,23 +662,30 @@ contains
 
     ! output_uu (mhdoutput.f90:72-131): [int32 4][float32 t][int32 4] then the
     ! raw (nx,ny,nz,8) FP64 array at byte 13, primitives unless output_primitive is false
+    ! The payload (rho, rho u or u, B, e or p) is formed on the device and copied into pinned
+    ! memory by laps_stage_output; output_uu stages and writes at once.
     subroutine output_uu(n, t)
         integer, intent(in) :: n
         real(dp), intent(in) :: t
-        real(dp), allocatable :: w(:, :, :, :), wp(:, :, :, :)
+        call laps_check(laps_stage_output(ctx, merge(1_c_int, 0_c_int, output_primitive)), 'laps_stage_output')
+        call write_staged(n, t)
+    end subroutine output_uu
+
+    subroutine write_staged(n, t)
+        integer, intent(in) :: n
+        real(dp), intent(in) :: t
+        type(c_ptr) :: pw
+        real(dp), pointer :: w(:, :, :, :)
+        real(dp) :: ts, dts
+        integer(c_int64_t) :: ist
         integer :: u
-        allocate(w(nx, ny, nz, 8), wp(nx, ny, nz, 4))
-        call laps_check(laps_get_state(ctx, w, wp), 'laps_get_state')
-        if (output_primitive) then
-            w(:, :, :, 2:4) = wp(:, :, :, 1:3)
-            w(:, :, :, 8) = wp(:, :, :, 4)
-        endif
+        call laps_check(laps_staged_output(ctx, pw, ts, dts, ist), 'laps_staged_output')
+        call c_f_pointer(pw, w, [nx, ny, nz, 8])
         open(newunit=u, file=out_name(n), access='stream', form='unformatted', status='replace', action='write')
         write(u) 4_c_int32_t, real(t, 4), 4_c_int32_t
         write(u) w
         close(u)
-        deallocate(w, wp)
-    end subroutine output_uu
+    end subroutine write_staged
 
     ! read_restart (restart.f90:17-63): primitives of out{n_start}.dat; time is float32
     subroutine read_restart(n, t)

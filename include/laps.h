@@ -129,6 +129,17 @@ int laps_get_scalars(laps_ctx* ctx, double* time, double* dt, double* radius,
  * mhdrhs.f90:282-294). Either pointer may be NULL. */
 int laps_get_state(laps_ctx* ctx, double* uu, double* uu_prim);
 
+/* Output staging (the write side of output_uu, mhdoutput.f90:72-131; the restart files of
+ * restart.f90:17-63 have the same payload). laps_stage_output forms the real-space state of
+ * the current step on the device and queues its copy into a pinned host buffer of the context
+ * on a stream of its own, so that laps_step calls made after it run while the copy proceeds.
+ * The payload is the (nx,ny,nz,8) FP64 array output_uu writes: rho, rho u (or u when
+ * primitive != 0, output_primitive), B, e (or p). laps_staged_output waits for the copy and
+ * returns a pointer to that payload (valid until the next laps_stage_output or
+ * laps_destroy) with the step's time, dt and istep. */
+int laps_stage_output(laps_ctx* ctx, int primitive);
+int laps_staged_output(laps_ctx* ctx, const double** payload, double* time, double* dt, int64_t* istep);
+
 /* Spectral state uu_fourier (nx/2+1,ny,nz,8), interleaved complex. */
 int laps_get_spectral(laps_ctx* ctx, double* uhat);
 
@@ -184,14 +195,19 @@ int laps_get_stream(laps_ctx* ctx, void** stream);
  * (laps_set_state, laps_get_*, diagnostics, laps_set_fixed_dt) applies to it,
  * and each box keeps its own CFL dt, time and stop status. laps_ens_step
  * advances every box by nsteps RK3 steps: a first step after any box's
- * laps_set_state runs per box, then one step of all boxes is captured as a
- * hipGraph (boxes as parallel branches) and every further step is one
- * launch of it. Like laps_step it is asynchronous; per-box reads
+ * laps_set_state runs per box; then the steps of all boxes are captured as
+ * hipGraphs of 8 steps and of 1 step (each launch group a branch, one launch
+ * per kernel for all boxes of the group) and replayed, 8 steps per launch
+ * while at least 8 remain. Like laps_step it is asynchronous; per-box reads
  * synchronise with it. laps_ens_destroy destroys the boxes too. */
 typedef struct laps_ens laps_ens;
 int laps_ens_create(int nbox, const laps_params* params, laps_ens** ens);
 void laps_ens_destroy(laps_ens* ens);
 int laps_ens_size(const laps_ens* ens);
+/* Number of launch groups: boxes with identical parameters are stepped together, one launch
+ * per kernel for the whole group (their step buffers sit at a fixed stride in pools of the
+ * ensemble); a box whose parameters match no other is a group of one. */
+int laps_ens_groups(const laps_ens* ens);
 laps_ctx* laps_ens_box(laps_ens* ens, int i);
 int laps_ens_step(laps_ens* ens, int nsteps);
 int laps_ens_sync(laps_ens* ens);

@@ -246,27 +246,23 @@ struct LdsTable {
     }
 };
 
-// Full line transform up to the last stage's butterfly. `load(p)` supplies
-// input position p and `lds(p)` is the LDS address of position p of this
-// work-item's line. `pre()` runs on every work-item after the first stage's
-// loads are issued and before the first barrier. On return, if j < BLAST, v[r]
-// holds output position j + r*BLAST (r < RLAST). Every work-item of the block
-// must call this. `tw` may point to LDS or HBM.
-template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds, class Pre = NoPre>
-__device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds,
-                                              cpx<T>* v, Pre pre = {}) {
+// Stages of a line transform whose first-stage inputs are already in registers: v[r] holds
+// input position j + r*B0 (r < R0) for j < B0. `after0()` runs on every work-item once the
+// first stage has written its results to LDS and before the first barrier: a kernel can issue
+// the loads of its next transform there, so that they are in flight during this one's LDS
+// stages. On return, if j < BLAST, v[r] holds output position j + r*BLAST (r < RLAST). Every
+// work-item of the block must call this. `tw` may point to LDS or HBM.
+template <int N, int DIR, typename T, class P = Plan<N>, class Lds, class After0 = NoPre>
+__device__ __forceinline__ bool line_fft_stages(int j, const cpx<T>* __restrict__ tw, Lds lds, cpx<T>* v,
+                                                After0 after0 = {}) {
     using I = PlanInfo<N, P>;
-    // stage 0 (span 1): HBM -> registers -> LDS
-    if (j < I::B0) {
-#pragma unroll
-        for (int r = 0; r < P::R0; ++r) v[r] = load(j + r * I::B0);
-    }
-    pre();
+    // stage 0 (span 1): registers -> LDS
     if (j < I::B0) {
         stage_bfly<N, P::R0, 1, DIR, T>(v, j, tw);
 #pragma unroll
         for (int r = 0; r < P::R0; ++r) *lds(stage_out<1, P::R0>(j, r)) = v[r];
     }
+    after0();
     __syncthreads();
     if constexpr (P::NST == 3) {
         constexpr int NS1 = P::R0;
@@ -297,6 +293,24 @@ __device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ 
         }
         return j < I::B1;
     }
+}
+
+// Full line transform up to the last stage's butterfly. `load(p)` supplies
+// input position p and `lds(p)` is the LDS address of position p of this
+// work-item's line. `pre()` runs on every work-item after the first stage's
+// loads are issued and before the first barrier. On return, if j < BLAST, v[r]
+// holds output position j + r*BLAST (r < RLAST). Every work-item of the block
+// must call this. `tw` may point to LDS or HBM.
+template <int N, int DIR, typename T, class P = Plan<N>, class Load, class Lds, class Pre = NoPre>
+__device__ __forceinline__ bool line_fft_regs(int j, const cpx<T>* __restrict__ tw, Load load, Lds lds,
+                                              cpx<T>* v, Pre pre = {}) {
+    using I = PlanInfo<N, P>;
+    if (j < I::B0) {
+#pragma unroll
+        for (int r = 0; r < P::R0; ++r) v[r] = load(j + r * I::B0);
+    }
+    pre();
+    return line_fft_stages<N, DIR, T, P>(j, tw, lds, v);
 }
 
 // Full line transform; `store(p, v)` consumes output position p.

@@ -19,9 +19,12 @@ namespace laps {
 template <typename T, int N, class P = Plan<N>>
 struct StridedCfg {
     static constexpr int TPL = PlanInfo<N, P>::TPL;
-    static constexpr int TX = (N >= 512) ? (sizeof(T) == 8 ? 4 : 8) : ((sizeof(T) == 4 && TPL <= 16) ? 32 : 16);
+    // 128-byte rows at least (8 FP64 / 16 FP32 columns at 512, whose three-stage plan has 64
+    // work-items per line: 512 or 1024 per block); 256-byte rows below 512
+    static constexpr int TX = (N >= 512) ? (sizeof(T) == 8 ? 8 : 16) : ((sizeof(T) == 4 && TPL <= 16) ? 32 : 16);
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;  // odd line stride: adjacent lines hit different banks
+    static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
 template <typename TI, typename T, int N, int DIR, class P>
@@ -33,6 +36,8 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
     __shared__ cpx<T> sh[C::TX * C::LS];
     __shared__ cpx<T> stw[N];
     long long bid = blockIdx.x;
+    in += blockIdx.y * a.in_bx;    // batched boxes: one per grid row
+    out += blockIdx.y * a.out_bx;
     const cpx<double>* ph = a.ph;
     if (a.nch == 2) {
         int ch;
@@ -74,6 +79,9 @@ __global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx
         const int my = (a.cy_cnt == a.mY) ? (b1 <= a.cy_cnt / 2 ? b1 : b1 - a.cy_cnt)
                                           : (b1 <= (a.cy_cnt - 1) / 2 ? b1 : b1 - a.cy_cnt);
         dead = mask_row_dead(a.mask, a.mX, a.mY, a.mZ, col, my);
+        // every wave holds all TX columns of the tile (TX divides 64): a tile with no live row
+        // is skipped whole, before any barrier (about a quarter of the tiles of a mask)
+        if (__ballot(valid && !dead) == 0) return;
     }
     auto load = [&](int p) -> cpx<T> {
         const int s = pos_to_stored(p, N, in_cnt);
@@ -202,8 +210,8 @@ static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs
     const long long nrest = (long long)ntile * a.nb1 * a.nb2;
     if (nrest <= 0) return hipSuccess;
     const long long nblk = a.nch == 2 ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest;
-    hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, scale,
-                       ntile, nrest);
+    hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0, s,
+                       in, out, a, tw, scale, ntile, nrest);
     return hipGetLastError();
 }
 

@@ -30,11 +30,12 @@ struct FwdCfg {
     using P = std::conditional_t<(sizeof(T) == 4 && N == 256), Plan<N>, PlanLow<N>>;
     using I = PlanInfo<N, P>;
     // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns) in the y pass; the z
-    // pass keeps 8 FP32 columns (16 measured slower)
-    static constexpr int TX = (sizeof(T) == 8) ? (N >= 512 ? 4 : 8) : (ZP || N >= 512 ? 8 : (N == 256 ? 32 : 16));
+    // pass keeps 8 FP32 columns (16 measured slower at 256)
+    static constexpr int TX = (sizeof(T) == 8) ? 8 : (ZP ? 8 : (N == 256 ? 32 : 16));
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
+    static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
 // derivative factors k' (real; the spectral factor is i*k'), mhdrhs.f90:192-204
@@ -71,6 +72,9 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
     twt.put(stw);  // visible after the barrier that opens each transform
     wyt.put(swy);
     const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
+    in += blockIdx.y * a.in_bx;    // batched boxes: one per grid row
+    out += blockIdx.y * a.out_bx;
+    ctl += blockIdx.y;
     long long bid = blockIdx.x;
     const long long bpc = (long long)ntile * nz * a.ngroups;  // blocks per chain
     if (a.nch == 2 && bid >= bpc) {
@@ -219,7 +223,14 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
                                                                T scale) {
     using C = FwdCfg<T, N, true>;
     using I = typename C::I;
-    if (ctl->status != 0) return;  // uniform over the grid: the step was refused
+    // batched boxes: one per grid row
+    ctl += blockIdx.y;
+    D += blockIdx.y * a.d_bx;
+    U += blockIdx.y * a.u_bx;
+    Fp += blockIdx.y * a.fp_bx;
+    const cpx<T>* const Ds = a.Ds ? (const cpx<T>*)a.Ds + blockIdx.y * a.d_bx : nullptr;
+    cpx<float>* const Uf = a.Uf ? a.Uf + blockIdx.y * a.uf_bx : nullptr;
+    if (ctl->status != 0) return;  // uniform over the box: the step was refused
     __shared__ cpx<T> sh[C::TX * C::LS];
     __shared__ cpx<T> stw[N];
     __shared__ double swz[N];
@@ -243,6 +254,9 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
     const double wx = tb.wx[kx], wy = tb.wy[cy];
     const int lim = tb.kzlim[(long long)cy * a.KX + kx];
     const bool live = valid && (OPT != 1 || lim >= 0);
+    // every wave holds all TX columns of the tile: a tile with no kept mode has nothing to
+    // transform, read or write (unless the modes outside the kept set must be cleared)
+    if (OPT == 1 && !a.clear_outside && __ballot(live) == 0) return;
     cpx<T>* line = sh + l * C::LS;
     auto lds = [&](int p) { return line + p; };
     const cpx<T>* ib = D + (long long)cy * a.rs + kx;
@@ -283,9 +297,9 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
             return h;
         };
         bool has;
-        if (a.Ds) {
+        if (Ds) {
             // phase-shift dealiasing: fnl = (Z(D) + conj(ph) Z(Ds)) / 2, ph of the output mode
-            const cpx<T>* ibs = (const cpx<T>*)a.Ds + (long long)cy * a.rs + kx;
+            const cpx<T>* ibs = Ds + (long long)cy * a.rs + kx;
             const cpx<double> wcb = cmul(a.phc[kx], a.phb[cy]);
             has = terms(ibs);
             cpx<T> accs[I::RLAST];
@@ -302,8 +316,10 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
         } else {
             has = terms(ib);
         }
-        // stage fnl of this column in the LDS line, then update the kept modes in
-        // a rolled loop: adjacent work-items take adjacent kx (coalesced U, Fp)
+        // stage fnl of this column in the LDS line, then update the kept modes: adjacent
+        // work-items take adjacent kx (coalesced U, Fp). A work-item takes its modes CH at a
+        // time and issues the state and previous-RHS loads of all CH before updating any, so
+        // a group costs one memory latency instead of one per mode.
         __syncthreads();
         if (has) {
 #pragma unroll
@@ -311,60 +327,80 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
         }
         __syncthreads();
         if (!valid) return;
+        constexpr int NIT = (N + C::TPL - 1) / C::TPL;
+        constexpr int CH = NIT < 4 ? NIT : 4;
 #pragma unroll 1
-        for (int p = j; p < N; p += C::TPL) {
-            const int cz = pos_to_stored(p, N, a.CZ);
-            if (cz < 0) continue;
-            const long long mi = (long long)cz * a.CY * a.rs + mode0;
-            if constexpr (OPT == 1) {
-                // outside the kept set the new state is zero whatever U and Fp hold there:
-                // neither is read, and Fp there is never read again
-                const int mz = p <= N / 2 ? p : p - N;
-                if (mz > lim || -mz > lim) {
-                    if (a.clear_outside) {
-                        U[v * a.ufs + mi] = cpx<double>{0.0, 0.0};
-                        if (a.Uf) a.Uf[v * a.ufs + mi] = cpx<float>{0.0f, 0.0f};
+        for (int i0 = 0; i0 < NIT; i0 += CH) {
+            long long mi[CH];
+            int czq[CH];
+            bool act[CH];
+            cpx<double> uq[CH];
+            cpx<T> fq[CH];
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+                const int p = j + (i0 + q) * C::TPL;
+                const int cz = p < N ? pos_to_stored(p, N, a.CZ) : -1;
+                czq[q] = cz;
+                mi[q] = (long long)(cz < 0 ? 0 : cz) * a.CY * a.rs + mode0;
+                bool k = cz >= 0;
+                if constexpr (OPT == 1) {
+                    // outside the kept set the new state is zero whatever U and Fp hold there:
+                    // neither is read, and Fp there is never read again
+                    const int mz = p <= N / 2 ? p : p - N;
+                    if (k && (mz > lim || -mz > lim)) {
+                        if (a.clear_outside) {
+                            U[v * a.ufs + mi[q]] = cpx<double>{0.0, 0.0};
+                            if (Uf) Uf[v * a.ufs + mi[q]] = cpx<float>{0.0f, 0.0f};
+                        }
+                        k = false;
                     }
-                    continue;
                 }
+                act[q] = k;
+                uq[q] = k ? U[v * a.ufs + mi[q]] : cpx<double>{0.0, 0.0};
+                fq[q] = (k && irk != 0) ? Fp[v * a.ufs + mi[q]] : cpx<T>{T(0), T(0)};
             }
-            const double wz = swz[p];
-            const cpx<T> fv = *lds(p);
-            cpx<double> fnl{double(fv.x) * double(scale), double(fv.y) * double(scale)};
-            if (a.clos && v >= 1 && v <= 6) fnl = fnl + a.clos[(v - 1) * a.ufs + mi];
-            const cpx<double> u = U[v * a.ufs + mi];
-            if (pp.aeb && v < 7) {
-                const double fac = (v == 2 || v == 3) ? 3.0 : (v >= 5 ? 1.0 : 2.0);
-                fnl = fnl - cpx<double>{(u.x * fac) / tau, (u.y * fac) / tau};
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+                if (!act[q]) continue;
+                const int p = j + (i0 + q) * C::TPL;
+                const int cz = czq[q];
+                const double wz = swz[p];
+                const cpx<T> fv = *lds(p);
+                cpx<double> fnl{double(fv.x) * double(scale), double(fv.y) * double(scale)};
+                if (a.clos && v >= 1 && v <= 6) fnl = fnl + a.clos[(v - 1) * a.ufs + mi[q]];
+                const cpx<double> u = uq[q];
+                if (pp.aeb && v < 7) {
+                    const double fac = (v == 2 || v == 3) ? 3.0 : (v >= 5 ? 1.0 : 2.0);
+                    fnl = fnl - cpx<double>{(u.x * fac) / tau, (u.y * fac) / tau};
+                }
+                const double k2 = (pp.visc || pp.resis) ? ksquare(wx, wy, wz, pp, ctl, radius) : 0.0;
+                if (pp.visc && pp.visc_exp && v >= 1 && v <= 3)
+                    fnl = fnl - cpx<double>{u.x * pp.nu * k2, u.y * pp.nu * k2};
+                if (pp.resis && pp.resis_exp && v >= 4 && v <= 6 && !(pp.conserve_bg && kx == 0 && p == 0))
+                    fnl = fnl - cpx<double>{u.x * pp.eta * k2, u.y * pp.eta * k2};
+                cpx<double> un;
+                if (irk == 0) {
+                    un = cpx<double>{c * fnl.x + 0.0, c * fnl.y + 0.0} + u;
+                } else {
+                    const cpx<double> fp{double(fq[q].x), double(fq[q].y)};
+                    un = cpx<double>{c * fnl.x + d * fp.x, c * fnl.y + d * fp.y} + u;
+                }
+                Fp[v * a.ufs + mi[q]] = cpx<T>{T(fnl.x), T(fnl.y)};
+                if (pp.visc && !pp.visc_exp && v >= 1 && v <= 3) {
+                    const double den = ts * k2 * pp.nu + 1.0;
+                    un = {un.x / den, un.y / den};
+                }
+                if (pp.resis && !pp.resis_exp && v >= 4 && v <= 6) {
+                    const double den = ts * k2 * pp.eta + 1.0;
+                    un = {un.x / den, un.y / den};
+                }
+                if constexpr (OPT == 2) {
+                    const double fx = tb.fx[kx], fy = tb.fy[cy], fz = tb.fz[cz];
+                    un = {((un.x * fx) * fy) * fz, ((un.y * fx) * fy) * fz};
+                }
+                U[v * a.ufs + mi[q]] = un;
+                if (Uf) Uf[v * a.ufs + mi[q]] = cpx<float>{float(un.x), float(un.y)};
             }
-            const double k2 = (pp.visc || pp.resis) ? ksquare(wx, wy, wz, pp, ctl, radius) : 0.0;
-            if (pp.visc && pp.visc_exp && v >= 1 && v <= 3)
-                fnl = fnl - cpx<double>{u.x * pp.nu * k2, u.y * pp.nu * k2};
-            if (pp.resis && pp.resis_exp && v >= 4 && v <= 6 && !(pp.conserve_bg && kx == 0 && p == 0))
-                fnl = fnl - cpx<double>{u.x * pp.eta * k2, u.y * pp.eta * k2};
-            cpx<double> un;
-            if (irk == 0) {
-                un = cpx<double>{c * fnl.x + 0.0, c * fnl.y + 0.0} + u;
-            } else {
-                const cpx<T> fpt = Fp[v * a.ufs + mi];
-                const cpx<double> fp{double(fpt.x), double(fpt.y)};
-                un = cpx<double>{c * fnl.x + d * fp.x, c * fnl.y + d * fp.y} + u;
-            }
-            Fp[v * a.ufs + mi] = cpx<T>{T(fnl.x), T(fnl.y)};
-            if (pp.visc && !pp.visc_exp && v >= 1 && v <= 3) {
-                const double den = ts * k2 * pp.nu + 1.0;
-                un = {un.x / den, un.y / den};
-            }
-            if (pp.resis && !pp.resis_exp && v >= 4 && v <= 6) {
-                const double den = ts * k2 * pp.eta + 1.0;
-                un = {un.x / den, un.y / den};
-            }
-            if constexpr (OPT == 2) {
-                const double fx = tb.fx[kx], fy = tb.fy[cy], fz = tb.fz[cz];
-                un = {((un.x * fx) * fy) * fz, ((un.y * fx) * fy) * fz};
-            }
-            U[v * a.ufs + mi] = un;
-            if (a.Uf) a.Uf[v * a.ufs + mi] = cpx<float>{float(un.x), float(un.y)};
         }
     }
 }
@@ -377,8 +413,8 @@ static hipError_t do_yg(const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cp
     using C = FwdCfg<T, N, false>;
     const int ntile = (a.ncol + C::TX - 1) / C::TX;
     const long long nblk = (long long)ntile * nz * a.ngroups * (a.nch == 2 ? 2 : 1);
-    hipLaunchKernelGGL((k_yfwd_group<T, N>), dim3((unsigned)nblk), dim3(C::NT), 0, s, in, out, a, tw, wxt, wyf, pp,
-                       ctl, ntile, nz);
+    hipLaunchKernelGGL((k_yfwd_group<T, N>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0, s, in, out,
+                       a, tw, wxt, wyf, pp, ctl, ntile, nz);
     return hipGetLastError();
 }
 
@@ -405,8 +441,8 @@ static hipError_t do_zr(const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, const ZArgs
     using C = FwdCfg<T, N, true>;
     const int ntile = (a.KX + C::TX - 1) / C::TX;
     const long long nblk = 8LL * ntile * a.CY;
-    hipLaunchKernelGGL((k_zfwd_rhs<T, N, OPT>), dim3((unsigned)nblk), dim3(C::NT), 0, s, D, U, Fp, a, tw, tb, wzf, pp,
-                       ctl, irk, ntile, scale);
+    hipLaunchKernelGGL((k_zfwd_rhs<T, N, OPT>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0, s, D, U,
+                       Fp, a, tw, tb, wzf, pp, ctl, irk, ntile, scale);
     return hipGetLastError();
 }
 

@@ -40,6 +40,10 @@ struct StridedArgs {
     // zero without loading them. The test is integer arithmetic on the grid lengths.
     int mask, row_role, cy_cnt;
     long long mX, mY, mZ;
+    // batched boxes (an ensemble group of identical contexts whose buffers sit at a fixed
+    // stride): the launch covers nbox boxes (grid y), box b at in + b * in_bx, out + b * out_bx
+    int nbox;
+    long long in_bx, out_bx;
 };
 
 // true when no mode of row (mx, my) is kept: its mz = 0 mode is outside the mask (the
@@ -138,6 +142,10 @@ struct XFArgs {
     // same launch, without the CFL reduction (0 or 1 counts as one chain)
     int nch;
     long long st_ch, jb_ch, out_ch;
+    // batched boxes (grid y): box b at st + b * st_bx, jb + b * jb_bx, out + b * out_bx and
+    // control block ctl + b
+    int nbox;
+    long long st_bx, jb_bx, out_bx;
 };
 template <typename T>
 hipError_t launch_xfused(int nx, bool first, bool cfl, const XFArgs& a, const cpx<T>* tw, const PhysParams& pp,
@@ -163,6 +171,9 @@ struct YGArgs {
     // optional [ncol]: output rows (kx, my) with |my| > ylim[kx] hold no kept mode and
     // are not stored (the z pass does not read them)
     const int* ylim;
+    // batched boxes (grid y): box b at in + b * in_bx, out + b * out_bx, control block ctl + b
+    int nbox;
+    long long in_bx, out_bx;
 };
 struct ZTerm {
     int field;
@@ -187,6 +198,10 @@ struct ZArgs {
     // with a mask, write zero to the modes outside the kept set (U, Uf): needed once after a
     // state upload, which leaves un-dealiased values there; later passes find them zero
     int clear_outside;
+    // batched boxes (grid y): box b at D/Ds + b * d_bx, U + b * u_bx, Fp + b * fp_bx,
+    // Uf + b * uf_bx, control block ctl + b
+    int nbox;
+    long long d_bx, u_bx, fp_bx, uf_bx;
 };
 template <typename T>
 hipError_t launch_yfwd_group(int ny, const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw,
@@ -198,13 +213,17 @@ hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, 
                            const Ctl* ctl, int irk, T scale, hipStream_t s);
 
 // --- physics launchers (physics_kernels.hip) ---
+// The step's small kernels take nbox batched boxes (ensemble groups): box b's state at
+// B + b * b_bx, current at J + b * j_bx, control block ctl + b.
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fstride, Box b, BoxTables t,
-                       const PhysParams& pp, const Ctl* ctl, hipStream_t s);
+                       const PhysParams& pp, const Ctl* ctl, hipStream_t s, int nbox = 1, long long b_bx = 0,
+                       long long j_bx = 0);
 hipError_t launch_curl(const cpx<double>* B, cpx<float>* J, long long fstride, Box b, BoxTables t,
-                       const PhysParams& pp, const Ctl* ctl, hipStream_t s);
-hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s);
-hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s);
-hipError_t launch_ctl_end(Ctl* ctl, const PhysParams& pp, hipStream_t s);
+                       const PhysParams& pp, const Ctl* ctl, hipStream_t s, int nbox = 1, long long b_bx = 0,
+                       long long j_bx = 0);
+hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s, int nbox = 1);
+hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s, int nbox = 1);
+hipError_t launch_ctl_end(Ctl* ctl, const PhysParams& pp, hipStream_t s, int nbox = 1);
 template <typename T>
 hipError_t launch_prim(const T* R, double* P, long long npts, double gamma, hipStream_t s);
 hipError_t launch_cfl(const double* U8, const double* P4, long long npts, const PhysParams& pp, Ctl* ctl,

@@ -169,6 +169,21 @@ struct laps_ctx {
     long long nkept = 0, nlive = 0;
     double fixed_dt = 0.0;
     int nblk_red = 1024;
+    // batched launches of an ensemble group, set on the group's first box while one step of the
+    // group is captured: n boxes whose U, Uf, Fp, P1, P2 sit at these element strides (in each
+    // buffer's own element type) from this box's, and whose control blocks follow this box's
+    struct Batch {
+        int n = 1;
+        long long u = 0, uf = 0, fp = 0, p1 = 0, p2 = 0;
+    } batch;
+    // output staging (laps_stage_output): the outNNN.dat payload (nx,ny,nz,8) copied from the
+    // real-space state S into pinned host memory on a stream of its own, while later steps run
+    double* stage_host = nullptr;       // hipHostMalloc'd, 8 N doubles
+    hipStream_t stage_stream = nullptr;
+    hipEvent_t stage_ready = nullptr, stage_done = nullptr;
+    bool stage_pending = false;          // a copy was queued: S is read until stage_done
+    double stage_time = 0.0, stage_dt = 0.0;
+    long long stage_istep = 0;
 
     int alloc(void** ptr, size_t bytes) {
         if (bytes == 0) bytes = 16;
@@ -386,7 +401,8 @@ const cpx<float>* tw_of<float>(laps_ctx* c, int axis) { return c->twf[axis]; }
 template <typename T, typename TS = double>
 int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
                  cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false, int nch = 1, long long a_ch = 0,
-                 long long b_ch = 0, bool masked = false) {
+                 long long b_ch = 0, bool masked = false, int nbox = 1, long long src_bx = 0, long long a_bx = 0,
+                 long long b_bx = 0) {
     if (nch == 2 && (dst || !shift)) return fail(LAPS_EINVAL, "internal: two chains are the shifted pair");
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
@@ -401,6 +417,7 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
     z.nb1 = b.CY; z.in_b1 = RS; z.out_b1 = RS;
     z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * RS;
     z.nch = nch; z.out_ch = a_ch; z.pair = true;
+    z.nbox = nbox; z.in_bx = src_bx; z.out_bx = a_bx;
     // masked: src is the kept-box state after a masked substage (zero outside the kept set);
     // the mask rows with no kept mode are skipped by the z inverse and read as zero by the y
     // inverse
@@ -419,6 +436,7 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
     y.nb1 = (int)NZ; y.in_b1 = (long long)b.CY * RS; y.out_b1 = NY * RS;
     y.nb2 = nf; y.in_b2 = NZ * b.CY * RS; y.out_b2 = NZ * NY * RS;
     y.nch = nch; y.in_ch = a_ch; y.out_ch = b_ch;
+    y.nbox = nbox; y.in_bx = a_bx; y.out_bx = b_bx;
     if (mask) {
         y.mask = mask; y.row_role = 2;
         y.mX = NX; y.mY = NY; y.mZ = NZ;
@@ -487,8 +505,16 @@ int substage(laps_ctx* c, int irk) {
     cpx<T>* P1 = (cpx<T>*)c->P1;
     cpx<T>* P2 = (cpx<T>*)c->P2;
     const Box& k = c->kept;
+    // batched launches (ensemble groups): every launch covers nb boxes
+    const laps_ctx::Batch& bt = c->batch;
+    const int nb = bt.n;
+    if (nb > 1 && first) return fail(LAPS_EINVAL, "internal: the first substage is not batched");
     int rc;
     XFArgs xa{};
+    xa.nbox = nb;
+    xa.st_bx = bt.p2;
+    xa.jb_bx = bt.p2;
+    xa.out_bx = bt.p1;
     xa.nrows = NZ * NY;
     xa.npts = c->N;
     xa.S = c->S;
@@ -519,11 +545,14 @@ int substage(laps_ctx* c, int irk) {
         if (hall)
             LAPS_CK(timed(c, K_CURL, (48.0 + 6.0 * sizeof(T)) * KB, [&] {
                 if (c->Uf)
-                    return launch_curl(c->U, c->Uf + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
-                return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream);
+                    return launch_curl(c->U, c->Uf + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream,
+                                       nb, bt.u, bt.uf);
+                return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream, nb,
+                                   bt.u, bt.u);
             }));
         if ((rc = inv_pipeline<T, T>(c, state_src<T>(c), KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv,
-                                     c->plan_yinv, nch == 2, nch, c->p1, c->p2, true)))
+                                     c->plan_yinv, nch == 2, nch, c->p1, c->p2, true, nb,
+                                     c->Uf ? bt.uf : bt.u, bt.p1, bt.p2)))
             return rc;
         xa.nch = nch;
         xa.st_ch = c->p2;
@@ -540,12 +569,12 @@ int substage(laps_ctx* c, int irk) {
         xin = (double)nch * c->nf_in() * xa.nrows * k.KX * 2 * sizeof(T);
     }
     const bool cfl = irk == 0 && !(c->fixed_dt > 0);
-    if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_begin(c->ctl, c->stream); }));
+    if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_begin(c->ctl, c->stream, nb); }));
     const double xout = (double)c->nf_out() * xa.nrows * k.KX * 2 * sizeof(T);
     LAPS_CK(timed(c, K_XFUSED, xin + nch * xout, [&] {
         return launch_xfused<T>(c->nx, first, cfl, xa, tw_of<T>(c, 0), c->pp, c->ctl, c->stream);
     }));
-    if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_finalize(c->ctl, c->pp, c->stream); }));
+    if (irk == 0) LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_finalize(c->ctl, c->pp, c->stream, nb); }));
     // y forward with the x/y derivative terms folded in, then z forward + RHS + RK3 + dealias
     const bool aeb = c->p.if_aeb != 0;
     YGArgs ya = aeb ? c->yg_aeb : c->yg;
@@ -553,6 +582,9 @@ int substage(laps_ctx* c, int irk) {
     ya.in_ch = c->p1;
     ya.out_ch = c->p2;
     ya.ylim = c->tkept.ylim;
+    ya.nbox = nb;
+    ya.in_bx = bt.p1;
+    ya.out_bx = bt.p2;
     LAPS_CK(timed(c, K_YFWD, nch * c->yfwd_bytes * sizeof(T), [&] {
         return launch_yfwd_group<T>(c->ny, P1, P2, ya, tw_of<T>(c, 1), c->tkept.wx, c->tfull.wy, c->pp, c->ctl,
                                     c->nz, c->stream);
@@ -585,6 +617,11 @@ int substage(laps_ctx* c, int irk) {
     }
     za.clos = c->closure ? c->cl_rhs : nullptr;
     za.Uf = c->Uf;
+    za.nbox = nb;
+    za.d_bx = bt.p2;
+    za.u_bx = bt.u;
+    za.fp_bx = bt.fp;
+    za.uf_bx = bt.uf;
     if (c->p.dealias_option == 3) {
         // the z pass averages the right-hand sides of the two grids
         za.Ds = P3;
@@ -683,7 +720,7 @@ int one_step(laps_ctx* c) {
     if (c->closure && (rc = closure_step(c))) return rc;
     for (int irk = 0; irk < 3; ++irk)
         if ((rc = substage<T>(c, irk))) return rc;
-    LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_end(c->ctl, c->pp, c->stream); }));
+    LAPS_CK(timed(c, K_CTL, 0.0, [&] { return launch_ctl_end(c->ctl, c->pp, c->stream, c->batch.n); }));
     c->fresh = false;
     c->materialized = false;
     return LAPS_OK;
@@ -719,11 +756,20 @@ int refresh_fresh(laps_ctx* c) {
     return LAPS_OK;
 }
 
+// S is read by a queued output copy until stage_done: work that rewrites S waits for it
+int wait_stage(laps_ctx* c, bool host) {
+    if (!c->stage_pending) return LAPS_OK;
+    if (host) LAPS_CK(hipEventSynchronize(c->stage_done));
+    else LAPS_CK(hipStreamWaitEvent(c->stream, c->stage_done, 0));
+    return LAPS_OK;
+}
+
 int materialize(laps_ctx* c) {
     int rc0 = refresh_fresh(c);
     if (rc0) return rc0;
     if (c->materialized) return LAPS_OK;
     int rc;
+    if ((rc = wait_stage(c, false))) return rc;
     if (c->cl_A) {
         // with the closure's FP64 scratch: the 8 fields in one batched transform
         TimingOff off(c);
@@ -924,6 +970,14 @@ void free_all(laps_ctx* c) {
     c->timing.pool.clear();
     for (void* ptr : c->allocs) (void)hipFree(ptr);
     c->allocs.clear();
+    if (c->stage_stream) (void)hipStreamSynchronize(c->stage_stream);
+    if (c->stage_host) (void)hipHostFree(c->stage_host);
+    if (c->stage_ready) (void)hipEventDestroy(c->stage_ready);
+    if (c->stage_done) (void)hipEventDestroy(c->stage_done);
+    if (c->stage_stream) (void)hipStreamDestroy(c->stage_stream);
+    c->stage_host = nullptr;
+    c->stage_ready = c->stage_done = nullptr;
+    c->stage_stream = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
@@ -1081,6 +1135,7 @@ int laps_set_state(laps_ctx* c, const double* uu, const double* uu_prim) {
     if (rc) return rc;
     if (!uu || !uu_prim) return fail(LAPS_EINVAL, "null state pointer");
     LAPS_CK(hipStreamSynchronize(c->stream));
+    if ((rc = wait_stage(c, true))) return rc;
     LAPS_CK(hipMemcpy(c->S, uu, 8 * c->N * sizeof(double), hipMemcpyHostToDevice));
     LAPS_CK(hipMemcpy(c->S + 8 * c->N, uu_prim, 4 * c->N * sizeof(double), hipMemcpyHostToDevice));
     for (int f = 0; f < 8; ++f)
@@ -1190,6 +1245,58 @@ int laps_get_state(laps_ctx* c, double* uu, double* uu_prim) {
     LAPS_CK(hipStreamSynchronize(c->stream));
     if (uu) LAPS_CK(hipMemcpy(uu, c->S, 8 * c->N * sizeof(double), hipMemcpyDeviceToHost));
     if (uu_prim) LAPS_CK(hipMemcpy(uu_prim, c->S + 8 * c->N, 4 * c->N * sizeof(double), hipMemcpyDeviceToHost));
+    return LAPS_OK;
+}
+
+int laps_stage_output(laps_ctx* c, int primitive) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!c->stage_host) {
+        if (hipHostMalloc((void**)&c->stage_host, 8 * c->N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            c->stage_host = nullptr;
+            return fail(LAPS_ENOMEM, "pinned output buffer allocation failed");
+        }
+        if (hipStreamCreateWithFlags(&c->stage_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->stage_ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess)
+            return fail(LAPS_EDEVICE, "output staging stream / events");
+    }
+    // the previous copy must have left the pinned buffer before this one overwrites it
+    if ((rc = wait_stage(c, true))) return rc;
+    if ((rc = materialize(c))) return rc;
+    Ctl h;
+    if ((rc = read_ctl(c, h))) return rc;   // the step scalars of this state (one small read)
+    c->stage_time = h.time;
+    c->stage_dt = h.dt;
+    c->stage_istep = h.istep;
+    // the payload of output_uu (mhdoutput.f90:87-117): rho, then rho u or u, B, then e or p,
+    // field blocks of S = [uu(8), uu_prim(4)] copied straight into the pinned buffer
+    LAPS_CK(hipEventRecord(c->stage_ready, c->stream));
+    LAPS_CK(hipStreamWaitEvent(c->stage_stream, c->stage_ready, 0));
+    const size_t fb = (size_t)c->N * sizeof(double);
+    const double* S = c->S;
+    double* H = c->stage_host;
+    const long long N = c->N;
+    LAPS_CK(hipMemcpyAsync(H, S, fb, hipMemcpyDeviceToHost, c->stage_stream));
+    LAPS_CK(hipMemcpyAsync(H + N, primitive ? S + 8 * N : S + N, 3 * fb, hipMemcpyDeviceToHost, c->stage_stream));
+    LAPS_CK(hipMemcpyAsync(H + 4 * N, S + 4 * N, 3 * fb, hipMemcpyDeviceToHost, c->stage_stream));
+    LAPS_CK(hipMemcpyAsync(H + 7 * N, primitive ? S + 11 * N : S + 7 * N, fb, hipMemcpyDeviceToHost,
+                           c->stage_stream));
+    LAPS_CK(hipEventRecord(c->stage_done, c->stage_stream));
+    c->stage_pending = true;
+    return LAPS_OK;
+}
+
+int laps_staged_output(laps_ctx* c, const double** payload, double* time, double* dt, int64_t* istep) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (!payload) return fail(LAPS_EINVAL, "null output");
+    if (!c->stage_pending) return fail(LAPS_ESTATE, "no output staged: call laps_stage_output first");
+    LAPS_CK(hipEventSynchronize(c->stage_done));
+    *payload = c->stage_host;
+    if (time) *time = c->stage_time;
+    if (dt) *dt = c->stage_dt;
+    if (istep) *istep = c->stage_istep;
     return LAPS_OK;
 }
 
@@ -1382,38 +1489,66 @@ int laps_get_stream(laps_ctx* c, void** stream) {
 // laps_ens_step capture again.
 struct laps_ens {
     std::vector<laps_ctx*> box;
+    // launch groups: boxes with identical parameters, whose step buffers (U, Uf, Fp, P1, P2 and
+    // the control block) sit at a fixed stride in pools of the ensemble, so that one launch per
+    // kernel steps the whole group; group[g][0] is the box whose context issues the launches
+    std::vector<std::vector<int>> group;
+    std::vector<laps_ctx::Batch> gbatch;
+    std::vector<void*> pools;
     std::vector<uint64_t> gen;         // box generations the graph was captured at
     int device = 0;
     hipStream_t origin = nullptr;
     hipEvent_t fork = nullptr, after = nullptr;  // capture fork; order after launches
     std::vector<hipEvent_t> join, before;        // capture joins; order before launches
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // graphs of 1 and of kEnsMulti steps: a launch of the latter advances every box by kEnsMulti
+    // steps (each box's dt and time live on the device, so the steps need no host work between)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
+constexpr int kEnsMulti = 8;
 
 static void ens_drop_graph(laps_ens* e) {
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
+    for (int g = 0; g < 2; ++g) {
+        if (e->exec[g]) (void)hipGraphExecDestroy(e->exec[g]);
+        if (e->graph[g]) (void)hipGraphDestroy(e->graph[g]);
+        e->exec[g] = nullptr;
+        e->graph[g] = nullptr;
+    }
 }
 
-static int ens_capture(laps_ens* e) {
-    ens_drop_graph(e);
+// captures graph `which` (0: one step, 1: kEnsMulti steps) of every box
+static int ens_capture(laps_ens* e, int which) {
+    const int nst = which ? kEnsMulti : 1;
     for (laps_ctx* c : e->box)
         if (c->closure) return fail(LAPS_EINVAL, "ensemble boxes with a learned closure are stepped per box");
     for (laps_ctx* c : e->box) c->timing.on = false;
     LAPS_CK(hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
     int rc = LAPS_OK;
     hipError_t he = hipEventRecord(e->fork, e->origin);
-    for (size_t i = 0; i < e->box.size() && he == hipSuccess && rc == LAPS_OK; ++i) {
-        laps_ctx* c = e->box[i];
-        he = hipStreamWaitEvent(c->stream, e->fork, 0);
-        if (he != hipSuccess) break;
-        rc = c->mixed ? one_step<float>(c) : one_step<double>(c);
-        if (rc) break;
-        he = hipEventRecord(e->join[i], c->stream);
-        if (he == hipSuccess) he = hipStreamWaitEvent(e->origin, e->join[i], 0);
+    for (size_t g = 0; g < e->group.size() && he == hipSuccess && rc == LAPS_OK; ++g) {
+        // a group whose boxes launch alike (same plans, same dt mode) is stepped by batched
+        // launches of its first box; otherwise each box is a branch of its own
+        const std::vector<int>& idx = e->group[g];
+        laps_ctx* c0 = e->box[idx[0]];
+        bool alike = idx.size() > 1;
+        for (int i : idx) {
+            const laps_ctx* c = e->box[i];
+            alike = alike && c->plan_zinv == c0->plan_zinv && c->plan_yinv == c0->plan_yinv &&
+                    c->fixed_dt == c0->fixed_dt && !c->fresh;
+        }
+        for (size_t m = 0; m < (alike ? 1 : idx.size()) && he == hipSuccess && rc == LAPS_OK; ++m) {
+            const int i = idx[m];
+            laps_ctx* c = e->box[i];
+            he = hipStreamWaitEvent(c->stream, e->fork, 0);
+            if (he != hipSuccess) break;
+            if (alike) c->batch = e->gbatch[g];
+            for (int st = 0; st < nst && rc == LAPS_OK; ++st)
+                rc = c->mixed ? one_step<float>(c) : one_step<double>(c);
+            c->batch = laps_ctx::Batch{};
+            if (rc) break;
+            he = hipEventRecord(e->join[i], c->stream);
+            if (he == hipSuccess) he = hipStreamWaitEvent(e->origin, e->join[i], 0);
+        }
     }
     hipGraph_t g = nullptr;
     const hipError_t he2 = hipStreamEndCapture(e->origin, &g);
@@ -1423,9 +1558,85 @@ static int ens_capture(laps_ens* e) {
         return fail(LAPS_EDEVICE, std::string("ensemble graph capture: ") +
                                       hipGetErrorString(he != hipSuccess ? he : he2));
     }
-    e->graph = g;
-    LAPS_CK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
-    for (size_t i = 0; i < e->box.size(); ++i) e->gen[i] = e->box[i]->gen;
+    e->graph[which] = g;
+    LAPS_CK(hipGraphInstantiate(&e->exec[which], e->graph[which], nullptr, nullptr, 0));
+    return LAPS_OK;
+}
+
+// Groups the boxes of identical parameters and moves each group's step buffers into pools of
+// the ensemble, box m of a group at m times a fixed stride, so that a launch of the group's
+// first box with grid rows for the boxes steps them all. The boxes have no state yet: nothing
+// is copied. Each box remains a full context of its own (per-box calls use its pointers).
+static int ens_make_groups(laps_ens* e) {
+    const int nbox = (int)e->box.size();
+    std::vector<bool> taken(nbox, false);
+    auto round256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    for (int i = 0; i < nbox; ++i) {
+        if (taken[i]) continue;
+        std::vector<int> idx{i};
+        taken[i] = true;
+        for (int j = i + 1; j < nbox; ++j)
+            if (!taken[j] && std::memcmp(&e->box[i]->p, &e->box[j]->p, sizeof(laps_params)) == 0) {
+                idx.push_back(j);
+                taken[j] = true;
+            }
+        e->group.push_back(idx);
+        e->gbatch.push_back(laps_ctx::Batch{});
+        const int G = (int)idx.size();
+        if (G < 2) continue;
+        laps_ctx* c0 = e->box[i];
+        const long long KB = c0->kb();
+        const size_t ts = c0->mixed ? sizeof(float) : sizeof(double);
+        const long long nchain = c0->p.dealias_option == 3 ? 2 : 1;
+        const size_t bu = round256(11 * KB * sizeof(cpx<double>));
+        const size_t buf = c0->Uf ? round256(11 * KB * sizeof(cpx<float>)) : 0;
+        const size_t bfp = round256(8 * KB * sizeof(cpx<double>));  // as laps_create sizes Fp (cleared whole)
+        const size_t bp1 = round256(nchain * c0->p1 * 2 * ts), bp2 = round256(nchain * c0->p2 * 2 * ts);
+        const size_t bctl = sizeof(Ctl);
+        void *pu = nullptr, *puf = nullptr, *pfp = nullptr, *pp1 = nullptr, *pp2 = nullptr, *pctl = nullptr;
+        auto pool = [&](void** q, size_t bytes) -> int {
+            if (bytes == 0) return LAPS_OK;
+            const hipError_t he = hipMalloc(q, bytes * G);
+            if (he != hipSuccess) {
+                *q = nullptr;
+                return fail(LAPS_ENOMEM, "ensemble pool allocation failed");
+            }
+            e->pools.push_back(*q);
+            return LAPS_OK;
+        };
+        int rc;
+        if ((rc = pool(&pu, bu)) || (rc = pool(&puf, buf)) || (rc = pool(&pfp, bfp)) || (rc = pool(&pp1, bp1)) ||
+            (rc = pool(&pp2, bp2)) || (rc = pool(&pctl, bctl)))
+            return rc;
+        if (puf && hipMemset(puf, 0, buf * G) != hipSuccess) return fail(LAPS_EDEVICE, "clearing a pool failed");
+        for (int m = 0; m < G; ++m) {
+            laps_ctx* c = e->box[idx[m]];
+            auto move = [&](void* old, void* fresh) {
+                for (auto it = c->allocs.begin(); it != c->allocs.end(); ++it)
+                    if (*it == old) {
+                        (void)hipFree(old);
+                        c->allocs.erase(it);
+                        break;
+                    }
+                return fresh;
+            };
+            c->U = (cpx<double>*)move(c->U, (char*)pu + m * bu);
+            if (c->Uf) c->Uf = (cpx<float>*)move(c->Uf, (char*)puf + m * buf);
+            c->Fp = (cpx<double>*)move(c->Fp, (char*)pfp + m * bfp);
+            c->P1 = move(c->P1, (char*)pp1 + m * bp1);
+            c->P2 = move(c->P2, (char*)pp2 + m * bp2);
+            if (c->P3) c->P3 = (char*)c->P2 + c->p2 * 2 * ts;
+            c->ctl = (Ctl*)move(c->ctl, (char*)pctl + m * bctl);
+            if ((rc = reset_ctl(c, 0.0, 0.0, false))) return rc;
+        }
+        laps_ctx::Batch& b = e->gbatch.back();
+        b.n = G;
+        b.u = (long long)(bu / sizeof(cpx<double>));
+        b.uf = (long long)(buf / sizeof(cpx<float>));
+        b.fp = (long long)(bfp / (2 * ts));
+        b.p1 = (long long)(bp1 / (2 * ts));
+        b.p2 = (long long)(bp2 / (2 * ts));
+    }
     return LAPS_OK;
 }
 
@@ -1448,6 +1659,10 @@ int laps_ens_create(int nbox, const laps_params* params, laps_ens** out) {
         e->box.push_back(c);
     }
     e->gen.assign(nbox, ~0ull);
+    {
+        const int rc = ens_make_groups(e);
+        if (rc) return bail(rc);
+    }
     if (hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&e->after, hipEventDisableTiming) != hipSuccess)
@@ -1475,10 +1690,13 @@ void laps_ens_destroy(laps_ens* e) {
     if (e->after) (void)hipEventDestroy(e->after);
     if (e->origin) (void)hipStreamDestroy(e->origin);
     for (laps_ctx* c : e->box) laps_destroy(c);
+    for (void* q : e->pools) (void)hipFree(q);  // the boxes' step buffers (no longer theirs to free)
     delete e;
 }
 
 int laps_ens_size(const laps_ens* e) { return e ? (int)e->box.size() : 0; }
+
+int laps_ens_groups(const laps_ens* e) { return e ? (int)e->group.size() : 0; }
 
 laps_ctx* laps_ens_box(laps_ens* e, int i) {
     if (!e || i < 0 || i >= (int)e->box.size()) return nullptr;
@@ -1497,26 +1715,43 @@ int laps_ens_step(laps_ens* e, int nsteps) {
     for (laps_ctx* c : e->box) eager = eager || c->fresh || !c->tuned;
     int rc;
     if (eager) {
-        for (laps_ctx* c : e->box)
-            if ((rc = laps_step(c, 1))) return rc;
+        // the boxes of a group take the FFT plans of its first box, so that their launches
+        // can be batched
+        for (const std::vector<int>& idx : e->group) {
+            laps_ctx* c0 = e->box[idx[0]];
+            for (size_t m = 0; m < idx.size(); ++m) {
+                laps_ctx* c = e->box[idx[m]];
+                if (m > 0 && c0->tuned && !c->tuned) {
+                    c->plan_zinv = c0->plan_zinv;
+                    c->plan_yinv = c0->plan_yinv;
+                    c->tuned = true;
+                }
+                if ((rc = laps_step(c, 1))) return rc;
+            }
+        }
         if (--nsteps == 0) return LAPS_OK;
     }
-    bool stale = !e->exec;
+    bool stale = false;
     for (size_t i = 0; i < e->box.size(); ++i) stale = stale || e->gen[i] != e->box[i]->gen;
+    if (stale) ens_drop_graph(e);
     // the graph's branches start after all work already queued on the box streams
     for (size_t i = 0; i < e->box.size(); ++i) {
         LAPS_CK(hipEventRecord(e->before[i], e->box[i]->stream));
         LAPS_CK(hipStreamWaitEvent(e->origin, e->before[i], 0));
     }
-    if (stale) {
-        // capturing records one step without running it; its launches are the graph's
+    const int nmulti = nsteps / kEnsMulti, nsingle = nsteps % kEnsMulti;
+    for (int which = 0; which < 2; ++which) {
+        if (e->exec[which] || (which ? nmulti : nsingle) == 0) continue;
+        // capturing records the steps without running them; their launches are the graph's
         std::vector<bool> timing(e->box.size());
         for (size_t i = 0; i < e->box.size(); ++i) timing[i] = e->box[i]->timing.on;
-        rc = ens_capture(e);
+        rc = ens_capture(e, which);
         for (size_t i = 0; i < e->box.size(); ++i) e->box[i]->timing.on = timing[i];
         if (rc) return rc;
     }
-    for (int s = 0; s < nsteps; ++s) LAPS_CK(hipGraphLaunch(e->exec, e->origin));
+    for (size_t i = 0; i < e->box.size(); ++i) e->gen[i] = e->box[i]->gen;
+    for (int s = 0; s < nmulti; ++s) LAPS_CK(hipGraphLaunch(e->exec[1], e->origin));
+    for (int s = 0; s < nsingle; ++s) LAPS_CK(hipGraphLaunch(e->exec[0], e->origin));
     LAPS_CK(hipEventRecord(e->after, e->origin));
     for (laps_ctx* c : e->box) {
         LAPS_CK(hipStreamWaitEvent(c->stream, e->after, 0));

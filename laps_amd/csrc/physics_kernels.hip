@@ -53,7 +53,11 @@ __device__ __forceinline__ bool box_index(long long idx, const Box& b, int& kx, 
 // J is written in FP64, or rounded once to FP32 for the mixed-precision step's z-inverse input
 template <typename TJ>
 __global__ void k_curl(const cpx<double>* __restrict__ U, cpx<TJ>* __restrict__ J, long long fs, Box b,
-                       BoxTables t, PhysParams pp, const Ctl* __restrict__ ctl) {
+                       BoxTables t, PhysParams pp, const Ctl* __restrict__ ctl, long long b_bx, long long j_bx) {
+    // batched boxes: one per grid row
+    U += blockIdx.y * b_bx;
+    J += blockIdx.y * j_bx;
+    ctl += blockIdx.y;
     const long long n = b.modes();
     const double radius = ctl->radius;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -70,14 +74,16 @@ __global__ void k_curl(const cpx<double>* __restrict__ U, cpx<TJ>* __restrict__ 
 }
 
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fs, Box b, BoxTables t, const PhysParams& pp,
-                       const Ctl* ctl, hipStream_t s) {
-    hipLaunchKernelGGL(k_curl<double>, dim3(grid_for(b.modes())), dim3(kThreads), 0, s, B, J, fs, b, t, pp, ctl);
+                       const Ctl* ctl, hipStream_t s, int nbox, long long b_bx, long long j_bx) {
+    hipLaunchKernelGGL(k_curl<double>, dim3(grid_for(b.modes()), nbox > 1 ? nbox : 1), dim3(kThreads), 0, s, B, J, fs, b,
+                       t, pp, ctl, b_bx, j_bx);
     return hipGetLastError();
 }
 
 hipError_t launch_curl(const cpx<double>* B, cpx<float>* J, long long fs, Box b, BoxTables t, const PhysParams& pp,
-                       const Ctl* ctl, hipStream_t s) {
-    hipLaunchKernelGGL(k_curl<float>, dim3(grid_for(b.modes())), dim3(kThreads), 0, s, B, J, fs, b, t, pp, ctl);
+                       const Ctl* ctl, hipStream_t s, int nbox, long long b_bx, long long j_bx) {
+    hipLaunchKernelGGL(k_curl<float>, dim3(grid_for(b.modes()), nbox > 1 ? nbox : 1), dim3(kThreads), 0, s, B, J, fs, b,
+                       t, pp, ctl, b_bx, j_bx);
     return hipGetLastError();
 }
 
@@ -99,12 +105,14 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* b
 
 // ------------------------------------------------------------------ control block
 __global__ void k_ctl_begin(Ctl* ctl) {
+    ctl += blockIdx.x;  // one block per batched box
     ctl->dtmin_slot[threadIdx.x] = 0x7FF0000000000000ULL;  // +Inf
     if (threadIdx.x == 0) ctl->dtmin_bits = 0x7FF0000000000000ULL;
 }
 
 // vardt tail + rkt_init + the dt < 1e-8 stop test (mhd.f90:216, :418-428; rktmod.f90:15-32)
 __global__ void k_ctl_finalize(Ctl* ctl, PhysParams pp) {
+    ctl += blockIdx.x;  // one block per batched box
     if (ctl->status != 0) return;
     double dt;
     if (pp.fixed_dt > 0) {
@@ -136,6 +144,7 @@ __global__ void k_ctl_finalize(Ctl* ctl, PhysParams pp) {
 
 // time += dt, istep += 1, evolve_radius(time) (mhd.f90:246-248; AEBmod.f90:56-73)
 __global__ void k_ctl_end(Ctl* ctl, PhysParams pp) {
+    ctl += blockIdx.x;  // one block per batched box
     if (ctl->status != 0) return;
     ctl->time = ctl->time + ctl->dt;
     ctl->istep += 1;
@@ -145,16 +154,16 @@ __global__ void k_ctl_end(Ctl* ctl, PhysParams pp) {
     ctl->ksq_updated = 1;
 }
 
-hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s) {
-    hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(kCflSlots), 0, s, ctl);
+hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s, int nbox) {
+    hipLaunchKernelGGL(k_ctl_begin, dim3(nbox > 1 ? nbox : 1), dim3(kCflSlots), 0, s, ctl);
     return hipGetLastError();
 }
-hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s) {
-    hipLaunchKernelGGL(k_ctl_finalize, dim3(1), dim3(1), 0, s, ctl, pp);
+hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s, int nbox) {
+    hipLaunchKernelGGL(k_ctl_finalize, dim3(nbox > 1 ? nbox : 1), dim3(1), 0, s, ctl, pp);
     return hipGetLastError();
 }
-hipError_t launch_ctl_end(Ctl* ctl, const PhysParams& pp, hipStream_t s) {
-    hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(1), 0, s, ctl, pp);
+hipError_t launch_ctl_end(Ctl* ctl, const PhysParams& pp, hipStream_t s, int nbox) {
+    hipLaunchKernelGGL(k_ctl_end, dim3(nbox > 1 ? nbox : 1), dim3(1), 0, s, ctl, pp);
     return hipGetLastError();
 }
 

@@ -312,6 +312,11 @@ __global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T
     const cpx<T>* tw = stw;
     const int lane = threadIdx.x % LR;
     const int rib = threadIdx.x / LR;
+    // batched boxes: one per grid row (offsets and control block of box blockIdx.y)
+    ctl += blockIdx.y;
+    a.st = (const cpx<T>*)a.st + blockIdx.y * a.st_bx;
+    a.jb = (const cpx<T>*)a.jb + blockIdx.y * a.jb_bx;
+    a.out = (cpx<T>*)a.out + blockIdx.y * a.out_bx;
     // two chains: the blocks of chain 1 follow those of chain 0
     long long blk = blockIdx.x;
     const long long gpc = (a.nrows + RPB - 1) / RPB;
@@ -464,10 +469,11 @@ static void xf_aeb(bool aeb, const XFArgs& a, const cpx<T>* tw, const PhysParams
     constexpr int NT = LaneFft<N>::NT;
     constexpr int RPB = NT / LaneFft<N>::LR;
     const unsigned g = (unsigned)((a.nrows + RPB - 1) / RPB * (a.nch == 2 ? 2 : 1));
+    const dim3 grid(g, a.nbox > 1 ? a.nbox : 1);
     if (aeb)
-        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, true>), dim3(g), dim3(NT), 0, s, a, tw, pp, ctl);
+        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, true>), grid, dim3(NT), 0, s, a, tw, pp, ctl);
     else
-        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, false>), dim3(g), dim3(NT), 0, s, a, tw, pp, ctl);
+        hipLaunchKernelGGL((k_xfused<T, N, FIRST, CFL, HALL, false>), grid, dim3(NT), 0, s, a, tw, pp, ctl);
 }
 
 template <typename T, int N, bool FIRST, bool CFL>

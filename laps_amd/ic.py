@@ -101,3 +101,29 @@ def cons_from_prim(prim, gamma):
         uu[..., 0] * (pr[..., 0] ** 2 + pr[..., 1] ** 2 + pr[..., 2] ** 2)
         + uu[..., 4] ** 2 + uu[..., 5] ** 2 + uu[..., 6] ** 2)
     return uu, pr
+
+
+def band_limited_state(n, m, gamma, fft_c2r, **seeded_kw):
+    """Conserved and primitive fields (uu, uu_prim) on the n^3 grid of a seeded state whose
+    modes all lie in shells up to kmax <= m/2 - 1: seeded_primitive on m^3 (same hash, same
+    rms), its spectrum zero-padded to n^3 and brought to real space by `fft_c2r` (a context's
+    laps_fft_c2r on n^3, seconds instead of minutes of numpy at 512^3). rho = p = 1 and B0 =
+    (1, 0, 0), as seeded_primitive's defaults; the fluctuations of u and B are transformed."""
+    pc = seeded_primitive((m, m, m), **seeded_kw)
+    idx = np.arange(m)
+    idx = np.where(idx <= m // 2, idx, idx - m) % n
+    spec = np.zeros((n // 2 + 1, n, n, 6), dtype=np.complex128, order="F")
+    for f, (col, off) in enumerate(((1, 0.0), (2, 0.0), (3, 0.0), (4, 1.0), (5, 0.0), (6, 0.0))):
+        F = np.fft.rfftn(pc[..., col] - off, axes=(2, 1, 0)) / m ** 3
+        spec[np.ix_(np.arange(m // 2 + 1), idx, idx, [f])] = F[..., None]
+    del pc
+    real = fft_c2r(spec)
+    del spec
+    prim = np.zeros((n, n, n, 8), order="F")
+    prim[..., 0] = 1.0
+    prim[..., 7] = 1.0
+    prim[..., 1:4] = real[..., 0:3]
+    prim[..., 4:7] = real[..., 3:6]
+    prim[..., 4] += 1.0
+    del real
+    return cons_from_prim(prim, gamma)

@@ -10,6 +10,12 @@ import math
 LAPS_FP64 = 0
 LAPS_MIXED = 1
 
+LAPS_OK = 0
+LAPS_EINVAL = -1
+LAPS_ENOMEM = -2
+LAPS_EDEVICE = -3
+LAPS_ESTATE = -4
+
 LAPS_RUNNING = 0
 LAPS_DT_STOP = 1
 

@@ -21,9 +21,10 @@ _D = ctypes.POINTER(ctypes.c_double)
 EXPORTS = (
     "laps_default_params", "laps_create", "laps_destroy", "laps_last_error", "laps_set_state",
     "laps_set_time", "laps_set_fixed_dt", "laps_cfl", "laps_step", "laps_get_scalars", "laps_get_state",
+    "laps_stage_output", "laps_staged_output",
     "laps_get_spectral", "laps_moments", "laps_budgets", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
     "laps_timing_enable", "laps_timing_read", "laps_get_stream",
-    "laps_ens_create", "laps_ens_destroy", "laps_ens_size", "laps_ens_box", "laps_ens_step", "laps_ens_sync",
+    "laps_ens_create", "laps_ens_destroy", "laps_ens_size", "laps_ens_groups", "laps_ens_box", "laps_ens_step", "laps_ens_sync",
     "laps_closure_set", "laps_closure_features", "laps_closure_eval", "laps_mlp_forward", "laps_mlp_grad",
     "laps_mlp_train",
 )
@@ -57,6 +58,8 @@ def lib():
         L.laps_get_scalars.argtypes = [vp, _D, _D, _D, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]
         L.laps_get_state.argtypes = [vp, _D, _D]
         L.laps_get_spectral.argtypes = [vp, _D]
+        L.laps_stage_output.argtypes = [vp, ctypes.c_int]
+        L.laps_staged_output.argtypes = [vp, ctypes.POINTER(_D), _D, _D, ctypes.POINTER(ctypes.c_int64)]
         L.laps_moments.argtypes = [vp, _D, _D, _D]
         L.laps_max_divb.argtypes = [vp, _D]
         L.laps_budgets.argtypes = [vp, _D]
@@ -71,6 +74,7 @@ def lib():
         L.laps_ens_destroy.argtypes = [vp]
         L.laps_ens_destroy.restype = None
         L.laps_ens_size.argtypes = [vp]
+        L.laps_ens_groups.argtypes = [vp]
         L.laps_ens_box.argtypes = [vp, ctypes.c_int]
         L.laps_ens_box.restype = vp
         L.laps_ens_step.argtypes = [vp, ctypes.c_int]
@@ -203,6 +207,23 @@ class Laps:
         _check(lib().laps_get_state(self._h, _fptr(uu), _fptr(pr)))
         return uu, pr
 
+    def stage_output(self, primitive=True):
+        """queue the output payload of the current step for an asynchronous copy to pinned memory
+        (laps_stage_output); later steps run while it is copied"""
+        _check(lib().laps_stage_output(self._h, 1 if primitive else 0))
+
+    def staged_output(self, copy=True):
+        """the staged payload (nx, ny, nz, 8) and the time, dt and istep it belongs to; copy=False
+        gives a view of the pinned buffer, valid until the next stage_output or close"""
+        ptr = _D()
+        t, dt, ist = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _check(lib().laps_staged_output(self._h, ctypes.byref(ptr), ctypes.byref(t), ctypes.byref(dt),
+                                        ctypes.byref(ist)))
+        buf = np.ctypeslib.as_array(ptr, shape=(self.nx * self.ny * self.nz * 8,))
+        w = buf.reshape((self.nx, self.ny, self.nz, 8), order="F")
+        return (np.array(w, order="F") if copy else w,
+                t.value, dt.value, int(ist.value))
+
     def spectral(self):
         u = self._spec(8)
         _check(lib().laps_get_spectral(self._h, _fptr(u)))
@@ -287,8 +308,9 @@ class Laps:
 
 class Ensemble:
     """Independent boxes advanced together (laps_ens_*): box(i) is a Laps view of
-    box i; step(n) advances all of them, one hipGraph launch per step after the
-    first."""
+    box i; step(n) advances all of them: after the first step, hipGraph launches of 8
+    steps (and of 1 step for the rest), boxes of identical parameters batched into
+    shared launches."""
 
     def __init__(self, params_list):
         arr = (LapsParams * len(params_list))(*params_list)
@@ -299,6 +321,10 @@ class Ensemble:
 
     def __len__(self):
         return len(self.boxes)
+
+    def groups(self):
+        """number of launch groups (boxes of identical parameters step together)"""
+        return int(lib().laps_ens_groups(self._h))
 
     def box(self, i):
         return self.boxes[i]

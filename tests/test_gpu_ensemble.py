@@ -110,3 +110,62 @@ def test_ensemble_box_vs_oracle(rt, oracle_mod):
         assert ens.box(0).scalars()["dt"] == pytest.approx(o.scalars()["dt"], rel=1e-13)
     finally:
         ens.close()
+
+
+GROUPED = [
+    (dict(n=32, if_hall=1, ion_inertial_length=0.05), (31, 32, 33)),
+    (dict(n=32, precision=LAPS_MIXED, dealias_option=3, if_hall=1, ion_inertial_length=0.05, cfl=0.2), (34, 35)),
+    (dict(n=(64, 32, 32), if_aeb=1, if_corotating=1, corotating_angle=0.3, radius0=1.0, Ur0=0.5), (36, 37)),
+    (dict(n=(96, 32, 32), precision=LAPS_MIXED, if_hall=1, ion_inertial_length=0.05), (38,)),
+]
+
+
+def test_batched_groups_match_eager_steps(rt):
+    """Boxes of identical parameters are stepped by shared launches (one launch per kernel for
+    the group, the boxes as grid rows): every box must still equal the same box stepped alone,
+    also after one box of a group gets a new state and another a fixed dt (that group is then
+    captured box by box)."""
+    ps, states = [], []
+    for kw, seeds in GROUPED:
+        kw = dict(kw)
+        n = kw.pop("n")
+        for sd in seeds:
+            p = params(n, **kw)
+            ps.append(p)
+            states.append(seeded_state(p, seed=sd, kmax=6, du=0.2, db=0.2))
+    ens = rt.Ensemble(ps)
+    solo = [rt.Laps(p) for p in ps]
+    try:
+        assert ens.groups() == len(GROUPED)
+        for i, (uu, pr) in enumerate(states):
+            ens.box(i).set_state(uu, pr)
+            solo[i].set_state(uu, pr)
+        for k in range(3):
+            ens.step(1)
+            for s in solo:
+                s.step(1)
+            print(f"batched step {k + 1}", flush=True)
+            check_equal(ens, solo, f"batched step {k + 1}")
+        ens.step(6)
+        for s in solo:
+            s.step(6)
+        check_equal(ens, solo, "6 more batched steps")
+        ens.step(17)   # two launches of the 8-step graph and one of the 1-step graph
+        for s in solo:
+            s.step(17)
+        check_equal(ens, solo, "17 more steps")
+        uu, pr = seeded_state(ps[1], seed=90, kmax=6, du=0.2, db=0.2)
+        ens.box(1).set_state(uu, pr)
+        solo[1].set_state(uu, pr)
+        ens.box(3).set_fixed_dt(2e-3)
+        solo[3].set_fixed_dt(2e-3)
+        ens.step(4)
+        for s in solo:
+            s.step(4)
+        check_equal(ens, solo, "after a new state and a fixed dt")
+        times = [ens.box(i).scalars()["time"] for i in range(len(ens))]
+        assert len(set(times)) > 3   # every box keeps its own CFL dt
+    finally:
+        ens.close()
+        for s in solo:
+            s.close()
