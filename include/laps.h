@@ -243,6 +243,18 @@ int laps_mlp_grad(int device, int64_t n, const float* X, const float* T, const f
 int laps_mlp_train(int device, int64_t n, const float* X, const float* T, float* params, int steps, int batch,
                    float lr, float* loss_hist);
 
+/* Gradient of the network without allocation. No counterpart in the reference; callers that
+ * cannot allocate device memory: code under HIP graph capture, tight training loops, harnesses
+ * that forbid hipMalloc. Same result as laps_mlp_grad, but X (n, 28), T (n, 15), params,
+ * grad (LAPS_MLP_NPARAM) and loss (1 float, may be NULL) are DEVICE pointers of `device`, and
+ * the call is asynchronous on `stream` (a hipStream_t; NULL: the null stream). The activations
+ * and deltas live in a fixed device array of the library (LAPS_MLP_NOALLOC_MAX samples, 9.4 MB
+ * on each device it runs on, held until the process ends): n above the limit is LAPS_EINVAL.
+ * Calls on one device share that array, so calls that overlap on one device race. */
+#define LAPS_MLP_NOALLOC_MAX 8192
+int laps_mlp_grad_noalloc(int device, int64_t n, const float* X, const float* T, const float* params,
+                          float* grad, float* loss, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

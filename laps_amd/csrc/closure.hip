@@ -314,6 +314,17 @@ hipError_t launch_mlp_grad(const float* X, const float* T, long long n, const fl
     return hipGetLastError();
 }
 
+// mean squared error from its sum over n samples of kClOut outputs, formed as the host side of
+// laps_mlp_grad forms it
+__global__ void k_loss_mean(const float* __restrict__ sum, long long n, float* __restrict__ out) {
+    if (threadIdx.x == 0) out[0] = sum[0] / (float)((double)n * kClOut);
+}
+
+hipError_t launch_loss_mean(const float* sum, long long n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss_mean, dim3(1), dim3(64), 0, s, sum, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_adam(float* P, const float* G, float* m, float* v, int n, float lr, float b1t, float b2t,
                        hipStream_t s) {
     hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(kThreadsCl), 0, s, P, G, m, v, n, lr, b1t, b2t);

@@ -253,6 +253,7 @@ hipError_t launch_mlp_fwd(const float* X, long long n, const float* P, float* Y,
                           hipStream_t s);
 hipError_t launch_mlp_grad(const float* X, const float* T, long long n, const float* P, const float* Y,
                            const float* H1, const float* H2, float* scratch, float* G, float* loss, hipStream_t s);
+hipError_t launch_loss_mean(const float* sum, long long n, float* out, hipStream_t s);
 hipError_t launch_adam(float* P, const float* G, float* m, float* v, int n, float lr, float b1t, float b2t,
                        hipStream_t s);
 hipError_t launch_cl_terms(const float* Y, long long N, const float* ystd, double gamma, double* C, hipStream_t s);

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1864,6 +1865,56 @@ int laps_mlp_grad(int device, int64_t n, const float* X, const float* T, const f
     float l = 0.f;
     LAPS_CK(hipMemcpy(&l, dL, sizeof(float), hipMemcpyDeviceToHost));
     if (loss) *loss = l / (float)((double)n * kClOut);
+    return LAPS_OK;
+}
+
+// ------------------------------------------------------------------ gradient without allocation
+// laps_mlp_grad_noalloc keeps the activations (Y 15, H1 64, H2 64 per sample) and the deltas
+// (dY 15, dZ2 64, dZ1 64, squared error 1 per sample) of the gradient in one file-scope device
+// array, so a call allocates nothing. Each device holds its own copy of the array; its address
+// is looked up once per device (hipGetSymbolAddress, with that device current) and only that
+// address is passed to kernels. Every call on a device shares the array: calls that overlap on
+// one device race.
+constexpr long long kNoAllocPerSample = 2LL * kClOut + 4LL * kClH + 1;
+constexpr long long kNoAllocFloats = (long long)LAPS_MLP_NOALLOC_MAX * kNoAllocPerSample + 64;
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+constexpr int kNoAllocDevices = 64;
+
+static int scratch_address(int device, float** out) {
+    static std::mutex mu;
+    static float* addr[kNoAllocDevices] = {};
+    if (device < 0 || device >= kNoAllocDevices) return fail(LAPS_EINVAL, "laps_mlp_grad_noalloc: bad device");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(LAPS_EDEVICE, std::string("laps_mlp_grad_noalloc: scratch address: ") + hipGetErrorString(e));
+        addr[device] = (float*)p;
+    }
+    *out = addr[device];
+    return LAPS_OK;
+}
+
+int laps_mlp_grad_noalloc(int device, int64_t n, const float* X, const float* T, const float* params,
+                          float* grad, float* loss, void* stream) {
+    if (n <= 0 || n > LAPS_MLP_NOALLOC_MAX)
+        return fail(LAPS_EINVAL, "laps_mlp_grad_noalloc: n = " + std::to_string((long long)n) +
+                                     " outside 1.." + std::to_string(LAPS_MLP_NOALLOC_MAX));
+    if (!X || !T || !params || !grad) return fail(LAPS_EINVAL, "laps_mlp_grad_noalloc: null buffer");
+    LAPS_CK(hipSetDevice(device));
+    float* sc = nullptr;
+    int rc;
+    if ((rc = scratch_address(device, &sc))) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    float* dY = sc;
+    float* dH1 = dY + n * kClOut;
+    float* dH2 = dH1 + n * kClH;
+    float* dS = dH2 + n * kClH;                                   // n * (2 kClH + kClOut + 1)
+    float* dsum = sc + (long long)LAPS_MLP_NOALLOC_MAX * kNoAllocPerSample;  // the squared-error sum
+    LAPS_CK(launch_mlp_fwd(X, n, params, dY, dH1, dH2, st));
+    LAPS_CK(launch_mlp_grad(X, T, n, params, dY, dH1, dH2, dS, grad, dsum, st));
+    if (loss) LAPS_CK(launch_loss_mean(dsum, n, loss, st));
     return LAPS_OK;
 }
 

@@ -26,7 +26,7 @@ EXPORTS = (
     "laps_timing_enable", "laps_timing_read", "laps_get_stream",
     "laps_ens_create", "laps_ens_destroy", "laps_ens_size", "laps_ens_groups", "laps_ens_box", "laps_ens_step", "laps_ens_sync",
     "laps_closure_set", "laps_closure_features", "laps_closure_eval", "laps_mlp_forward", "laps_mlp_grad",
-    "laps_mlp_train",
+    "laps_mlp_train", "laps_mlp_grad_noalloc",
 )
 
 MLP_NPARAM = 6991          # LAPS_MLP_NPARAM
@@ -86,6 +86,7 @@ def lib():
         L.laps_mlp_grad.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, _F, _F]
         L.laps_mlp_train.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_float, _F]
+        L.laps_mlp_grad_noalloc.argtypes = [ctypes.c_int, ctypes.c_int64] + [vp] * 6
         _lib = L
     return _lib
 
