@@ -217,26 +217,33 @@ int laps_ens_sync(laps_ens* ens);
  * that stand in for finer ones. Features per point (28): rho, u, B and
  * their central-difference gradients d/dx, d/dy, d/dz (rho, ux, uy, uz,
  * Bx, By, Bz), each (f - mean)/std. Network 28 -> 64 (tanh) -> 64 (tanh) ->
- * 15 in FP32: subgrid Reynolds stress (xx, xy, xz, yy, yz, zz), Maxwell
- * stress (same order) and EMF (x, y, z), each times ystd. Parameter vector:
- * W1[28][64], b1[64], W2[64][64], b2[64], W3[64][15], b3[15] (the network,
- * LAPS_MLP_NPARAM floats), then xmean[28], xstd[28], ystd[15].
- * laps_closure_set (NULL: off) makes every following step evaluate the
- * network on the state at the start of the step and add -div(R - M + q I)
- * to the momentum and -curl(EMF) to the induction right-hand side of its
- * three substages, with q the subgrid part of p + B^2/2. AEB is not
- * supported. laps_closure_features returns the raw (unnormalised) features
- * (nx*ny*nz, 28); laps_closure_eval the normalised outputs (nx*ny*nz, 15)
- * and, if rhs != NULL, the RHS term (nx/2+1, ny, nz, 6) of the current state.
- * laps_mlp_*: the network on caller samples (rows of 28 inputs, 15 targets)
+ * 18 in FP32: subgrid Reynolds stress (xx, xy, xz, yy, yz, zz), Maxwell
+ * stress (same order), EMF (x, y, z) and energy flux (x, y, z), each times
+ * ystd. Parameter vector: W1[28][64], b1[64], W2[64][64], b2[64], W3[64][18],
+ * b3[18] (the network, LAPS_MLP_NPARAM floats), then xmean[28], xstd[28],
+ * ystd[18]. laps_closure_set (NULL: off) makes every following step evaluate
+ * the network on the state at the start of the step and add -div(R - M + q I)
+ * to the momentum, -curl(EMF) to the induction and -div(energy flux) to the
+ * energy right-hand side of its three substages, with q the subgrid part of
+ * p + B^2/2. AEB is not supported. laps_closure_features returns the raw
+ * (unnormalised) features (nx*ny*nz, 28); laps_closure_eval the normalised
+ * outputs (nx*ny*nz, 18) and, if rhs != NULL, the RHS term (nx/2+1, ny, nz, 7:
+ * momentum, B, energy) of the current state.
+ * laps_mlp_*: the network on caller samples (rows of 28 inputs, 18 targets)
  * on device `device`: forward; gradient of the mean squared error with its
  * loss; `steps` Adam steps (lr, betas 0.9/0.999) over consecutive batches of
  * the (caller-shuffled) samples, with the loss of each batch. */
-#define LAPS_MLP_NPARAM 6991
-#define LAPS_CLOSURE_NPARAM (LAPS_MLP_NPARAM + 71)
+#define LAPS_MLP_NPARAM 7186
+#define LAPS_CLOSURE_NPARAM (LAPS_MLP_NPARAM + 74)
 int laps_closure_set(laps_ctx* ctx, const float* params, int nparams);
 int laps_closure_features(laps_ctx* ctx, float* features);
 int laps_closure_eval(laps_ctx* ctx, float* outputs, double* rhs);
+/* Training data of the closure, no counterpart in the reference: the box filter (mean over r^3
+ * fine cells, r dividing the grid) of the current state and of its flux products, formed on
+ * the device. out (nx/r, ny/r, nz/r, 26): rho, rho u (3), B (3), e; rho u_i u_j and B_i B_j
+ * (xx, xy, xz, yy, yz, zz each); E = -u x B + (d_i/rho) J x B (3, J = curl B, d_i = 0 without
+ * Hall); the energy flux (e + p + B^2/2) u - (u.B) B (3) (the fluxes of mhdrhs.f90:60-119). */
+int laps_closure_filter(laps_ctx* ctx, int r, double* out);
 int laps_mlp_forward(int device, int64_t n, const float* X, const float* params, float* Y);
 int laps_mlp_grad(int device, int64_t n, const float* X, const float* T, const float* params, float* grad,
                   float* loss);
@@ -245,10 +252,10 @@ int laps_mlp_train(int device, int64_t n, const float* X, const float* T, float*
 
 /* Gradient of the network without allocation. No counterpart in the reference; callers that
  * cannot allocate device memory: code under HIP graph capture, tight training loops, harnesses
- * that forbid hipMalloc. Same result as laps_mlp_grad, but X (n, 28), T (n, 15), params,
+ * that forbid hipMalloc. Same result as laps_mlp_grad, but X (n, 28), T (n, 18), params,
  * grad (LAPS_MLP_NPARAM) and loss (1 float, may be NULL) are DEVICE pointers of `device`, and
  * the call is asynchronous on `stream` (a hipStream_t; NULL: the null stream). The activations
- * and deltas live in a fixed device array of the library (LAPS_MLP_NOALLOC_MAX samples, 9.4 MB
+ * and deltas live in a fixed device array of the library (LAPS_MLP_NOALLOC_MAX samples, 9.6 MB
  * on each device it runs on, held until the process ends): n above the limit is LAPS_EINVAL.
  * Calls on one device share that array, so calls that overlap on one device race. */
 #define LAPS_MLP_NOALLOC_MAX 8192

@@ -4,20 +4,21 @@
 // No counterpart in the reference (its fluxes are the resolved ones only,
 // mhdrhs.f90:60-119). On a coarse grid the filtered equations carry
 // unresolved flux terms: the subgrid Reynolds stress R_ij, Maxwell stress
-// M_ij and electromotive force. A small network maps the resolved state at a
-// point and its 3^3-stencil gradients to those 15 terms; the step adds the
-// divergence of the momentum-flux correction and the curl of the EMF
-// correction to the right-hand side.
+// M_ij, electromotive force and energy flux. A small network maps the resolved
+// state at a point and its 3^3-stencil gradients to those 18 terms; the step
+// adds the divergence of the momentum-flux and energy-flux corrections and the
+// curl of the EMF correction to the right-hand side.
 //
 //   features (28): rho, u (3), B (3), then d/dx, d/dy, d/dz of rho, ux, uy,
 //                  uz, Bx, By, Bz (central differences, periodic), each
 //                  normalised as (f - mean) / std
-//   network:       28 -> 64 (tanh) -> 64 (tanh) -> 15, FP32
-//   outputs (15):  R xx, xy, xz, yy, yz, zz; M xx, xy, xz, yy, yz, zz;
-//                  EMF x, y, z; physical value = output * ystd
+//   network:       28 -> 64 (tanh) -> 64 (tanh) -> 18, FP32
+//   outputs (18):  R xx, xy, xz, yy, yz, zz; M xx, xy, xz, yy, yz, zz;
+//                  EMF x, y, z; energy flux x, y, z;
+//                  physical value = output * ystd
 //
 // Parameter vector (FP32): W1[28][64], b1[64], W2[64][64], b2[64],
-// W3[64][15], b3[15], then xmean[28], xstd[28], ystd[15].
+// W3[64][18], b3[18], then xmean[28], xstd[28], ystd[18].
 #include "internal.hpp"
 
 namespace laps {
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kThreadsCl) void k_mlp_fwd(const float* __restrict_
     }
 }
 
-// Mean-squared-error loss L = sum (Y - T)^2 / (n * 15): per-sample deltas
+// Mean-squared-error loss L = sum (Y - T)^2 / (n * 18): per-sample deltas
 // dY = dL/dY, dZ2 = dL/d(pre-activation 2), dZ1 = dL/d(pre-activation 1),
 // and the per-sample squared error for the loss.
 __global__ __launch_bounds__(kThreadsCl) void k_mlp_delta(const float* __restrict__ Y, const float* __restrict__ T,
@@ -211,11 +212,11 @@ __global__ void k_sum(const float* __restrict__ x, long long n, float* __restric
 
 // ---------------------------------------------------------------- closure terms
 
-// Network outputs (normalised) -> the 9 real correction fields of the step:
+// Network outputs (normalised) -> the 12 real correction fields of the step:
 // C[0..5] = momentum flux tau_ij = R_ij - M_ij + delta_ij q (xx, xy, xz, yy,
-// yz, zz) and C[6..8] = EMF correction. q is the subgrid part of the total
-// pressure p + B^2/2 with p = (gamma-1)(e - rho u.u/2 - B^2/2):
-// q = -(gamma-1)/2 (tr R + tr M) + tr M / 2.
+// yz, zz), C[6..8] = EMF correction, C[9..11] = energy-flux correction. q is
+// the subgrid part of the total pressure p + B^2/2 with
+// p = (gamma-1)(e - rho u.u/2 - B^2/2): q = -(gamma-1)/2 (tr R + tr M) + tr M / 2.
 __global__ void k_cl_terms(const float* __restrict__ Y, long long N, const float* __restrict__ ystd, double gamma,
                            double* __restrict__ C) {
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -228,14 +229,15 @@ __global__ void k_cl_terms(const float* __restrict__ Y, long long N, const float
 #pragma unroll
     for (int c = 0; c < 6; ++c) C[c * N + i] = o[c] - o[6 + c] + ((c == 0 || c == 3 || c == 5) ? q : 0.0);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) C[(6 + c) * N + i] = o[12 + c];
+    for (int c = 0; c < 6; ++c) C[(6 + c) * N + i] = o[12 + c];
 }
 
 // Spectral right-hand side of the correction fields (full spectra Ch, FP64,
-// layout [9][nz][ny][nxh]): momentum_i -= i k'_j tau_ij (as calc_rhs does for
-// the resolved momentum flux, mhdrhs.f90:206-222) and B -= i k' x E (the
-// induction term of the electric field E, mhdrhs.f90:224-233). Out: [6][M],
-// momentum x, y, z then B x, y, z.
+// layout [12][nz][ny][nxh]): momentum_i -= i k'_j tau_ij (as calc_rhs does for
+// the resolved momentum flux, mhdrhs.f90:206-222), B -= i k' x E (the
+// induction term of the electric field E, mhdrhs.f90:224-233) and e -= i k'.F
+// (the energy flux, mhdrhs.f90:234-236). Out: [7][M], momentum x, y, z, B x,
+// y, z, energy.
 __global__ void k_cl_rhs(const cpx<double>* __restrict__ Ch, long long M, int nxh, int ny,
                          const double* __restrict__ wx, const double* __restrict__ wy, const double* __restrict__ wz,
                          cpx<double>* __restrict__ R) {
@@ -268,6 +270,69 @@ __global__ void k_cl_rhs(const cpx<double>* __restrict__ Ch, long long M, int nx
         const double sx = k[b] * E[c].x - k[c] * E[b].x, sy = k[b] * E[c].y - k[c] * E[b].y;
         R[(3 + a) * M + i] = cpx<double>{sy, -sx};
     }
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const cpx<double> v = Ch[(9 + b) * M + i];
+        sx += k[b] * v.x;
+        sy += k[b] * v.y;
+    }
+    R[6 * M + i] = cpx<double>{sy, -sx};
+}
+
+// ---------------------------------------------------------------- training data
+
+// Box filter of a fine state and of its flux products, one work-item per coarse cell (mean over
+// r^3 fine cells): out[q][coarse cell], q = 0..7 rho, m x y z, B x y z, e; 8..13 m_i u_j and
+// 14..19 B_i B_j (xx, xy, xz, yy, yz, zz); 20..22 E = -u x B + (d_i/rho) J x B; 23..25 the
+// energy flux (e + p + B^2/2) u - (u.B) B (mhdrhs.f90:60-119). S = [uu(8), uu_prim(4)] and J
+// (3 fields) are the fine real fields, FP64.
+__global__ void k_cl_filter(const double* __restrict__ S, const double* __restrict__ J, int nx, int ny, int nz, int r,
+                            double di, double* __restrict__ out) {
+    const int cx = nx / r, cy = ny / r, cz = nz / r;
+    const long long Nc = (long long)cx * cy * cz;
+    const long long ic = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (ic >= Nc) return;
+    const int x0 = (int)(ic % cx) * r, y0 = (int)((ic / cx) % cy) * r, z0 = (int)(ic / ((long long)cx * cy)) * r;
+    const long long N = (long long)nx * ny * nz;
+    double acc[kClFilt];
+#pragma unroll
+    for (int q = 0; q < kClFilt; ++q) acc[q] = 0.0;
+    const int pr[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int dz = 0; dz < r; ++dz)
+        for (int dy = 0; dy < r; ++dy)
+            for (int dx = 0; dx < r; ++dx) {
+                const long long i = ((long long)(z0 + dz) * ny + (y0 + dy)) * nx + (x0 + dx);
+                double w[8], u[3], b[3], j[3];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w[q] = S[q * N + i];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    u[a] = S[(8 + a) * N + i];
+                    b[a] = w[4 + a];
+                    j[a] = J[a * N + i];
+                }
+                const double p = S[11 * N + i], rho = w[0];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[q] += w[q];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    acc[8 + c] += w[1 + pr[c][0]] * u[pr[c][1]];
+                    acc[14 + c] += b[pr[c][0]] * b[pr[c][1]];
+                }
+                const double h = di / rho;
+                const double ub = u[0] * b[0] + u[1] * b[1] + u[2] * b[2];
+                const double et = w[7] + p + 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int s = (a + 1) % 3, t = (a + 2) % 3;
+                    acc[20 + a] += -(u[s] * b[t] - u[t] * b[s]) + h * (j[s] * b[t] - j[t] * b[s]);
+                    acc[23 + a] += et * u[a] - ub * b[a];
+                }
+            }
+    const double inv = 1.0 / ((double)r * r * r);
+#pragma unroll
+    for (int q = 0; q < kClFilt; ++q) out[q * Nc + ic] = acc[q] * inv;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -279,6 +344,13 @@ hipError_t launch_cl_features(const double* S, int nx, int ny, int nz, double dx
     const long long N = (long long)nx * ny * nz;
     hipLaunchKernelGGL(k_cl_features, dim3(blocks_for(N)), dim3(kThreadsCl), 0, s, S, nx, ny, nz, dx, dy, dz, norm,
                        X);
+    return hipGetLastError();
+}
+
+hipError_t launch_cl_filter(const double* S, const double* J, int nx, int ny, int nz, int r, double di, double* out,
+                            hipStream_t s) {
+    const long long Nc = (long long)(nx / r) * (ny / r) * (nz / r);
+    hipLaunchKernelGGL(k_cl_filter, dim3(blocks_for(Nc)), dim3(kThreadsCl), 0, s, S, J, nx, ny, nz, r, di, out);
     return hipGetLastError();
 }
 

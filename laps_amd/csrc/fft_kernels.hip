@@ -16,23 +16,26 @@
 
 namespace laps {
 
-template <typename T, int N, class P = Plan<N>>
+template <typename T, int N, class P = Plan<N>, int W = 0>
 struct StridedCfg {
     static constexpr int TPL = PlanInfo<N, P>::TPL;
     // 128-byte rows at least (8 FP64 / 16 FP32 columns at 512, whose three-stage plan has 64
     // work-items per line: 512 or 1024 per block); 256-byte rows below 512
-    static constexpr int TX = (N >= 512) ? (sizeof(T) == 8 ? 8 : 16) : ((sizeof(T) == 4 && TPL <= 16) ? 32 : 16);
+    static constexpr int TX0 = (N >= 512) ? (sizeof(T) == 8 ? 8 : 16) : ((sizeof(T) == 4 && TPL <= 16) ? 32 : 16);
+    // W = 1: the other tile width the plan autotuner tries: half as many columns (smaller blocks,
+    // more of them per CU) where the default has 16 or more, else twice as many
+    static constexpr int TX = W == 0 ? TX0 : (TX0 >= 16 ? TX0 / 2 : 2 * TX0);
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;  // odd line stride: adjacent lines hit different banks
     static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
-template <typename TI, typename T, int N, int DIR, class P>
-__global__ __launch_bounds__((StridedCfg<T, N, P>::NT)) void k_strided(const cpx<TI>* __restrict__ in,
-                                                                  cpx<T>* __restrict__ out, StridedArgs a,
-                                                                  const cpx<T>* __restrict__ tw, T scale,
-                                                                  int ntile, long long nrest) {
-    using C = StridedCfg<T, N, P>;
+template <typename TI, typename T, int N, int DIR, class P, int W>
+__global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const cpx<TI>* __restrict__ in,
+                                                                     cpx<T>* __restrict__ out, StridedArgs a,
+                                                                     const cpx<T>* __restrict__ tw, T scale,
+                                                                     int ntile, long long nrest) {
+    using C = StridedCfg<T, N, P, W>;
     __shared__ cpx<T> sh[C::TX * C::LS];
     __shared__ cpx<T> stw[N];
     long long bid = blockIdx.x;
@@ -202,33 +205,41 @@ __global__ __launch_bounds__((XCfg<T, N>::NT)) void k_x_r2c(const T* __restrict_
 
 // ---------------------------------------------------------------- launchers
 
-template <typename TI, typename T, int N, int DIR, class P>
+template <typename TI, typename T, int N, int DIR, class P, int W>
 static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw, T scale,
                                hipStream_t s) {
-    using C = StridedCfg<T, N, P>;
+    using C = StridedCfg<T, N, P, W>;
     const int ntile = (a.ncol + C::TX - 1) / C::TX;
     const long long nrest = (long long)ntile * a.nb1 * a.nb2;
     if (nrest <= 0) return hipSuccess;
     const long long nblk = a.nch == 2 ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest;
-    hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0, s,
-                       in, out, a, tw, scale, ntile, nrest);
+    hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P, W>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0,
+                       s, in, out, a, tw, scale, ntile, nrest);
     return hipGetLastError();
 }
 
+// variant: bit 0 the plan (0 Plan, 1 PlanLow), bit 1 the tile width (inverse passes only)
 template <typename TI, typename T, int N, int DIR>
-static hipError_t do_strided(int plan, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw,
+static hipError_t do_strided(int variant, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw,
                              T scale, hipStream_t s) {
-    if (plan == 1) return do_strided_p<TI, T, N, DIR, PlanLow<N>>(in, out, a, tw, scale, s);
-    return do_strided_p<TI, T, N, DIR, Plan<N>>(in, out, a, tw, scale, s);
+    const bool low = variant & 1;
+    if constexpr (DIR > 0) {
+        if (variant & 2) {
+            if (low) return do_strided_p<TI, T, N, DIR, PlanLow<N>, 1>(in, out, a, tw, scale, s);
+            return do_strided_p<TI, T, N, DIR, Plan<N>, 1>(in, out, a, tw, scale, s);
+        }
+    }
+    if (low) return do_strided_p<TI, T, N, DIR, PlanLow<N>, 0>(in, out, a, tw, scale, s);
+    return do_strided_p<TI, T, N, DIR, Plan<N>, 0>(in, out, a, tw, scale, s);
 }
 
 template <typename TI, typename T>
 hipError_t launch_strided(int n, int dir, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw,
-                          T scale, hipStream_t s, int plan) {
+                          T scale, hipStream_t s, int variant) {
 #define LAPS_STRIDED_CASE(NN)                                                              \
     case NN:                                                                               \
-        return dir < 0 ? do_strided<TI, T, NN, -1>(plan, in, out, a, tw, scale, s)         \
-                       : do_strided<TI, T, NN, +1>(plan, in, out, a, tw, scale, s);
+        return dir < 0 ? do_strided<TI, T, NN, -1>(variant, in, out, a, tw, scale, s)      \
+                       : do_strided<TI, T, NN, +1>(variant, in, out, a, tw, scale, s);
     switch (n) {
         LAPS_STRIDED_CASE(16)
         LAPS_STRIDED_CASE(32)

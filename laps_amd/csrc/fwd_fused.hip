@@ -15,6 +15,7 @@
 // curls P5 = -i ky' Y(E_z), P6 = i kx' Y(E_z), P7 = i ky' Y(E_x) - i kx' Y(E_y).
 // With corotating AEB, kx' and ky' both depend on (kx, ky); both are known per
 // output mode of the y pass.
+
 #include <type_traits>
 
 #include "internal.hpp"
@@ -22,15 +23,18 @@
 
 namespace laps {
 
-// ZP: the z pass, whose blocks also carry the RHS and RK3 update. The FP32 passes at 256 use
-// the two-stage radix-16x16 plan (one LDS exchange; FP32 has the registers for 16 values per
-// work-item): y 3.09 -> 2.74 ms and z 3.42 -> 3.26 ms per phase-shift step.
+// ZP: the z pass, whose blocks also carry the RHS and RK3 update. The passes use the plan of
+// radices <= 8 (8 values per work-item, so a block keeps one transform's result live across
+// the next at 4-5 waves per SIMD), except the FP32 y pass at 256, which runs faster on the
+// two-stage radix-16x16 plan (one LDS exchange, 16 values per work-item: 2.62 against 3.15
+// ms per mixed phase-shift step). On that plan the FP32 z pass ran at 2 waves per SIMD
+// (236 registers) and took 2.83 ms; on the low-radix plan 2.74 ms.
 template <typename T, int N, bool ZP>
 struct FwdCfg {
-    using P = std::conditional_t<(sizeof(T) == 4 && N == 256), Plan<N>, PlanLow<N>>;
+    using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP), Plan<N>, PlanLow<N>>;
     using I = PlanInfo<N, P>;
-    // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns) in the y pass; the z
-    // pass keeps 8 FP32 columns (16 measured slower at 256)
+    // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns), 256 bytes for the FP32
+    // y pass on the two-stage plan; the z pass keeps 8 FP32 columns (16 measured slower at 256)
     static constexpr int TX = (sizeof(T) == 8) ? 8 : (ZP ? 8 : (N == 256 ? 32 : 16));
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
@@ -367,7 +371,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx
                 const double wz = swz[p];
                 const cpx<T> fv = *lds(p);
                 cpx<double> fnl{double(fv.x) * double(scale), double(fv.y) * double(scale)};
-                if (a.clos && v >= 1 && v <= 6) fnl = fnl + a.clos[(v - 1) * a.ufs + mi[q]];
+                if (a.clos && v >= 1) fnl = fnl + a.clos[(v - 1) * a.ufs + mi[q]];
                 const cpx<double> u = uq[q];
                 if (pp.aeb && v < 7) {
                     const double fac = (v == 2 || v == 3) ? 3.0 : (v >= 5 ? 1.0 : 2.0);

@@ -115,10 +115,12 @@ struct Twiddles {
 };
 
 // --- FFT launchers (fft_kernels.hip) ---
-// plan 0: two stages where possible (Plan), 1: radices <= 8 (PlanLow)
+// variant bit 0: 0 two stages where possible (Plan), 1 radices <= 8 (PlanLow); bit 1 (inverse
+// passes only): the other tile width (half the columns, or twice as many where the default has
+// fewer than 16)
 template <typename TI, typename T>
 hipError_t launch_strided(int n, int dir, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a,
-                          const cpx<T>* tw, T scale, hipStream_t s, int plan = 0);
+                          const cpx<T>* tw, T scale, hipStream_t s, int variant = 0);
 template <typename T>
 hipError_t launch_x_c2r(int n, const cpx<T>* in, T* out, const XArgs& a, const cpx<T>* tw, hipStream_t s);
 template <typename T>
@@ -193,7 +195,7 @@ struct ZArgs {
     const cpx<double>* ph;    // fnl = (Z(D) + conj(ph[p] phc[kx] phb[cy]) Z(Ds)) / 2 at output mode
     const cpx<double>* phc;   // (kx, cy, p)
     const cpx<double>* phb;
-    const cpx<double>* clos;  // optional learned-closure RHS [6][modes] added to momentum and B (v = 1..6)
+    const cpx<double>* clos;  // optional learned-closure RHS [7][modes] added to momentum, B and e (v = 1..7)
     cpx<float>* Uf;           // optional FP32 copy of the updated state (mixed precision)
     // with a mask, write zero to the modes outside the kept set (U, Uf): needed once after a
     // state upload, which leaves un-dealiased values there; later passes find them zero
@@ -244,9 +246,12 @@ template <typename TI, typename TO>
 hipError_t launch_convert(const TI* in, TO* out, long long n, hipStream_t s);
 
 // --- learned closure (closure.hip) ---
-constexpr int kClIn = 28, kClH = 64, kClOut = 15;
+constexpr int kClIn = 28, kClH = 64, kClOut = 18;
 constexpr int kClNParam = kClIn * kClH + kClH + kClH * kClH + kClH + kClH * kClOut + kClOut;
 constexpr int kClNNorm = 2 * kClIn + kClOut;  // xmean, xstd, ystd
+constexpr int kClFilt = 26;  // box-filtered fields of laps_closure_filter
+hipError_t launch_cl_filter(const double* S, const double* J, int nx, int ny, int nz, int r, double di, double* out,
+                            hipStream_t s);
 hipError_t launch_cl_features(const double* S, int nx, int ny, int nz, double dx, double dy, double dz,
                               const float* norm, float* X, hipStream_t s);
 hipError_t launch_mlp_fwd(const float* X, long long n, const float* P, float* Y, float* H1, float* H2,

@@ -161,7 +161,7 @@ struct laps_ctx {
     bool have_state = false, fresh = false, materialized = false, stepped_since_set = false;
     uint64_t gen = 0;  // bumped when the kernels a step launches change (state upload, fixed dt)
     bool tuned = false;
-    int plan_zinv = 0, plan_yinv = 0;  // FFT plans of the inverse z / y passes, chosen by autotune()
+    int plan_zinv = 0, plan_yinv = 0;  // FFT variants (plan, tile width) of the inverse z / y passes, by autotune()
     YGArgs yg{}, yg_aeb{};
     ZArgs zr{}, zr_aeb{};
     double yfwd_bytes = 0, zfwd_bytes = 0;  // algorithmic elements x 2 (multiply by sizeof(T))
@@ -646,12 +646,12 @@ int substage(laps_ctx* c, int irk) {
     return LAPS_OK;
 }
 
-// Times the inverse z and y passes of the kept box with both FFT plans
-// (events on the context stream, on the real buffers) and keeps the faster.
+// Times the inverse z and y passes of the kept box in each variant (two FFT plans x two tile
+// widths; events on the context stream, on the real buffers) and keeps the fastest of each.
 template <typename T>
 int autotune(laps_ctx* c) {
-    // LAPS_FFT_PLAN = 0 or 1 fixes both plans instead (reproducible runs: the timed choice
-    // can differ between contexts, and with it the rounding of the transforms)
+    // LAPS_FFT_PLAN = 0 or 1 fixes both plans (default tile width) instead (reproducible runs:
+    // the timed choice can differ between contexts, and with it the rounding of the transforms)
     if (const char* fixed = std::getenv("LAPS_FFT_PLAN")) {
         if ((fixed[0] == '0' || fixed[0] == '1') && fixed[1] == 0) {
             c->plan_zinv = c->plan_yinv = fixed[0] - '0';
@@ -666,50 +666,74 @@ int autotune(laps_ctx* c) {
     hipEvent_t e0, e1;
     LAPS_CK(hipEventCreate(&e0));
     LAPS_CK(hipEventCreate(&e1));
-    float best[2][2] = {{0, 0}, {0, 0}};  // [pass][plan]
+    constexpr int NV = 4;
+    float best_zy[NV] = {0, 0, 0, 0}, best_y[NV] = {0, 0, 0, 0};  // min over repetitions
     // the launches a step makes: with phase-shift dealiasing both grids as two chains
     const int nch = c->p.dealias_option == 3 ? 2 : 1;
-    for (int plan = 0; plan < 2; ++plan) {
-        for (int rep = 0; rep < 4; ++rep) {
-            // z+y with plan for the y part held at 0, then y alone with plan
-            LAPS_CK(hipEventRecord(e0, c->stream));
-            int rc = inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, plan, 0,
-                                        nch == 2, nch, c->p1, c->p2, true);
-            if (rc) return rc;
-            LAPS_CK(hipEventRecord(e1, c->stream));
-            LAPS_CK(hipEventSynchronize(e1));
+    const long long NY = c->ny, NZ = c->nz, RS = k.KXP;
+    StridedArgs y{};
+    y.ncol = k.KX; y.in_cnt = k.CY; y.out_cnt = (int)NY;
+    y.in_ls = RS; y.out_ls = RS;
+    y.nb1 = (int)NZ; y.in_b1 = (long long)k.CY * RS; y.out_b1 = NY * RS;
+    y.nb2 = c->nf_in(); y.in_b2 = NZ * k.CY * RS; y.out_b2 = NZ * NY * RS;
+    y.nch = nch; y.in_ch = c->p1; y.out_ch = c->p2;
+    if (c->p.dealias_option == 1 || c->p.dealias_option == 3) {  // as in the substage
+        y.mask = c->p.dealias_option; y.row_role = 2;
+        y.mX = c->nx; y.mY = NY; y.mZ = NZ;
+    }
+    auto time_it = [&](auto launch, float& ms) -> int {
+        LAPS_CK(hipEventRecord(e0, c->stream));
+        const int rc = launch();
+        if (rc) return rc;
+        LAPS_CK(hipEventRecord(e1, c->stream));
+        LAPS_CK(hipEventSynchronize(e1));
+        LAPS_CK(hipEventElapsedTime(&ms, e0, e1));
+        return LAPS_OK;
+    };
+    for (int rep = 0; rep < 4; ++rep) {
+        for (int v = 0; v < NV; ++v) {
+            // z (variant v) + y (variant 0), then y alone (variant v)
             float ms_zy = 0, ms_y = 0;
-            LAPS_CK(hipEventElapsedTime(&ms_zy, e0, e1));
-            const long long NY = c->ny, NZ = c->nz, RS = k.KXP;
-            StridedArgs y{};
-            y.ncol = k.KX; y.in_cnt = k.CY; y.out_cnt = (int)NY;
-            y.in_ls = RS; y.out_ls = RS;
-            y.nb1 = (int)NZ; y.in_b1 = (long long)k.CY * RS; y.out_b1 = NY * RS;
-            y.nb2 = c->nf_in(); y.in_b2 = NZ * k.CY * RS; y.out_b2 = NZ * NY * RS;
-            y.nch = nch; y.in_ch = c->p1; y.out_ch = c->p2;
-            if (c->p.dealias_option == 1 || c->p.dealias_option == 3) {  // as in the substage
-                y.mask = c->p.dealias_option; y.row_role = 2;
-                y.mX = c->nx; y.mY = NY; y.mZ = NZ;
+            int rc = time_it([&] {
+                return inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, v, 0,
+                                          nch == 2, nch, c->p1, c->p2, true);
+            }, ms_zy);
+            if (!rc) rc = time_it([&] {
+                const hipError_t he = launch_strided<T, T>(c->ny, +1, P1, P2, y, tw_of<T>(c, 1), T(1), c->stream, v);
+                return he == hipSuccess ? LAPS_OK : fail(LAPS_EDEVICE, hipGetErrorString(he));
+            }, ms_y);
+            if (rc) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return rc;
             }
-            LAPS_CK(hipEventRecord(e0, c->stream));
-            LAPS_CK(launch_strided<T, T>(c->ny, +1, P1, P2, y, tw_of<T>(c, 1), T(1), c->stream, plan));
-            LAPS_CK(hipEventRecord(e1, c->stream));
-            LAPS_CK(hipEventSynchronize(e1));
-            LAPS_CK(hipEventElapsedTime(&ms_y, e0, e1));
             if (rep == 0) continue;  // warm-up
-            best[1][plan] = (rep == 1 || ms_y < best[1][plan]) ? ms_y : best[1][plan];
-            const float z = ms_zy;   // z (with this plan) + y (plan 0)
-            best[0][plan] = (rep == 1 || z < best[0][plan]) ? z : best[0][plan];
+            best_zy[v] = (rep == 1 || ms_zy < best_zy[v]) ? ms_zy : best_zy[v];
+            best_y[v] = (rep == 1 || ms_y < best_y[v]) ? ms_y : best_y[v];
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // plan 1 only when clearly faster (3 %), so repeated runs pick the same code; the z pass
-    // is compared on its own time (z + y with the y plan held at 0, less that y time)
-    const float z0 = best[0][0] - best[1][0], z1 = best[0][1] - best[1][0];
-    c->plan_zinv = z1 < 0.97f * z0 ? 1 : 0;
-    c->plan_yinv = best[1][1] < 0.97f * best[1][0] ? 1 : 0;
+    // a variant other than the default only when clearly faster (3 %), so repeated runs pick
+    // the same code; the z pass is compared on its own time (z + y with the y variant held at 0,
+    // less that y time)
+    int vz = 0, vy = 0;
+    float tz = best_zy[0] - best_y[0], ty = best_y[0];
+    for (int v = 1; v < NV; ++v) {
+        const float z = best_zy[v] - best_y[0];
+        if (z < 0.97f * tz) { tz = z; vz = v; }
+        if (best_y[v] < 0.97f * ty) { ty = best_y[v]; vy = v; }
+    }
+    c->plan_zinv = vz;
+    c->plan_yinv = vy;
     c->tuned = true;
+    if (std::getenv("LAPS_TUNE_LOG")) {
+        std::fprintf(stderr, "[laps] autotune %dx%dx%d %s: z inverse", c->nx, c->ny, c->nz, c->mixed ? "mixed" : "fp64");
+        for (int v = 0; v < NV; ++v) std::fprintf(stderr, " %.3f", best_zy[v] - best_y[0]);
+        std::fprintf(stderr, " ms -> variant %d; y inverse", vz);
+        for (int v = 0; v < NV; ++v) std::fprintf(stderr, " %.3f", best_y[v]);
+        std::fprintf(stderr, " ms -> variant %d\n", vy);
+    }
     return LAPS_OK;
 }
 
@@ -793,10 +817,10 @@ int closure_buffers(laps_ctx* c) {
     int rc;
     if ((rc = c->allocn(&c->cl_P, kClNParam + kClNNorm)) || (rc = c->allocn(&c->cl_X, c->N * kClIn)) ||
         (rc = c->allocn(&c->cl_Y, c->N * kClOut)) || (rc = c->allocn(&c->cl_id, 2 * kClIn)) ||
-        (rc = c->allocn(&c->cl_C, 9 * c->N)) ||
-        (rc = c->allocn(&c->cl_A, std::max(9 * c->M, 8LL * c->nz * c->ny * c->kept.KXP))) ||
-        (rc = c->allocn(&c->cl_B, std::max(9 * c->M, 8LL * c->nz * c->ny * c->kept.KXP))) ||
-        (rc = c->allocn(&c->cl_full, 6 * c->M)) || (rc = c->allocn(&c->cl_rhs, 6 * c->kb())))
+        (rc = c->allocn(&c->cl_C, 12 * c->N)) ||
+        (rc = c->allocn(&c->cl_A, std::max(12 * c->M, 8LL * c->nz * c->ny * c->kept.KXP))) ||
+        (rc = c->allocn(&c->cl_B, std::max(12 * c->M, 8LL * c->nz * c->ny * c->kept.KXP))) ||
+        (rc = c->allocn(&c->cl_full, 7 * c->M)) || (rc = c->allocn(&c->cl_rhs, 7 * c->kb())))
         return rc;
     std::vector<float> id(2 * kClIn, 0.f);
     for (int q = 0; q < kClIn; ++q) id[kClIn + q] = 1.f;
@@ -818,12 +842,12 @@ int closure_step(laps_ctx* c) {
     LAPS_CK(launch_cl_terms(c->cl_Y, c->N, c->cl_P + kClNParam + 2 * kClIn, c->p.adiabatic_index, c->cl_C,
                             c->stream));
     {
-        TimingOff off(c);  // the 9 fields in one batched transform; spectra [9][M] in cl_A
-        if ((rc = fwd_pipeline<double>(c, c->cl_C, 9, c->full, c->cl_A, c->cl_B))) return rc;
+        TimingOff off(c);  // the 12 fields in one batched transform; spectra [12][M] in cl_A
+        if ((rc = fwd_pipeline<double>(c, c->cl_C, 12, c->full, c->cl_A, c->cl_B))) return rc;
     }
     LAPS_CK(launch_cl_rhs(c->cl_A, c->M, c->nxh, c->ny, c->tfull.wx, c->tfull.wy, c->tfull.wz, c->cl_full,
                           c->stream));
-    LAPS_CK(launch_gather_box(c->cl_full, c->cl_rhs, 6, c->M, c->kb(), c->nxh, c->ny, c->nz, c->kept, c->stream));
+    LAPS_CK(launch_gather_box(c->cl_full, c->cl_rhs, 7, c->M, c->kb(), c->nxh, c->ny, c->nz, c->kept, c->stream));
     return LAPS_OK;
 }
 
@@ -1802,6 +1826,43 @@ int laps_closure_features(laps_ctx* c, float* out) {
     return LAPS_OK;
 }
 
+int laps_closure_filter(laps_ctx* c, int r, double* out) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!out) return fail(LAPS_EINVAL, "null output");
+    if (r < 1 || c->nx % r || c->ny % r || c->nz % r) return fail(LAPS_EINVAL, "the filter width must divide the grid");
+    if ((rc = materialize(c))) return rc;
+    // J = curl B of the current state in real space: the spectral curl (the Hall-current slots of
+    // the state buffer, which every substage rewrites) and three FP64 inverse transforms
+    const bool fr = c->fresh;
+    cpx<double>* src = fr ? c->Ufull : c->U;
+    const Box& b = fr ? c->full : c->kept;
+    const long long fs = fr ? c->M : c->kb();
+    const long long Nc = (c->N / r / r) / r;
+    double *J = nullptr, *O = nullptr;
+    if (hipMalloc(&J, 3 * c->N * sizeof(double)) != hipSuccess) return fail(LAPS_ENOMEM, "current buffer");
+    if (hipMalloc(&O, kClFilt * Nc * sizeof(double)) != hipSuccess) {
+        (void)hipFree(J);
+        return fail(LAPS_ENOMEM, "filter output buffer");
+    }
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(J);
+        (void)hipFree(O);
+        return code;
+    };
+    hipError_t he = launch_curl(src, src + 8 * fs, fs, b, (fr ? c->tfull : c->tkept).view(), c->pp, c->ctl, c->stream);
+    if (he != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+    for (int a = 0; a < 3; ++a)
+        if ((rc = inv_fp64(c, src + (8 + a) * fs, b, J + a * c->N))) return done(rc);
+    he = launch_cl_filter(c->S, J, c->nx, c->ny, c->nz, r, c->p.if_hall ? c->p.ion_inertial_length : 0.0, O,
+                          c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he == hipSuccess) he = hipMemcpy(out, O, kClFilt * Nc * sizeof(double), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+    return done(LAPS_OK);
+}
+
 int laps_closure_eval(laps_ctx* c, float* out, double* rhs) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
@@ -1809,7 +1870,7 @@ int laps_closure_eval(laps_ctx* c, float* out, double* rhs) {
     if ((rc = closure_step(c))) return rc;
     LAPS_CK(hipStreamSynchronize(c->stream));
     if (out) LAPS_CK(hipMemcpy(out, c->cl_Y, (size_t)c->N * kClOut * sizeof(float), hipMemcpyDeviceToHost));
-    if (rhs) LAPS_CK(hipMemcpy(rhs, c->cl_full, (size_t)6 * c->M * sizeof(cpx<double>), hipMemcpyDeviceToHost));
+    if (rhs) LAPS_CK(hipMemcpy(rhs, c->cl_full, (size_t)7 * c->M * sizeof(cpx<double>), hipMemcpyDeviceToHost));
     return LAPS_OK;
 }
 

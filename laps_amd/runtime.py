@@ -25,12 +25,13 @@ EXPORTS = (
     "laps_get_spectral", "laps_moments", "laps_budgets", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
     "laps_timing_enable", "laps_timing_read", "laps_get_stream",
     "laps_ens_create", "laps_ens_destroy", "laps_ens_size", "laps_ens_groups", "laps_ens_box", "laps_ens_step", "laps_ens_sync",
-    "laps_closure_set", "laps_closure_features", "laps_closure_eval", "laps_mlp_forward", "laps_mlp_grad",
+    "laps_closure_set", "laps_closure_features", "laps_closure_eval", "laps_closure_filter", "laps_mlp_forward", "laps_mlp_grad",
     "laps_mlp_train", "laps_mlp_grad_noalloc",
 )
 
-MLP_NPARAM = 6991          # LAPS_MLP_NPARAM
-CLOSURE_NPARAM = 6991 + 71  # LAPS_CLOSURE_NPARAM
+MLP_NPARAM = 7186          # LAPS_MLP_NPARAM
+MLP_NIN, MLP_NOUT = 28, 18  # network inputs and outputs per sample
+CLOSURE_NPARAM = 7186 + 74  # LAPS_CLOSURE_NPARAM
 _F = ctypes.POINTER(ctypes.c_float)
 
 
@@ -82,6 +83,7 @@ def lib():
         L.laps_closure_set.argtypes = [vp, _F, ctypes.c_int]
         L.laps_closure_features.argtypes = [vp, _F]
         L.laps_closure_eval.argtypes = [vp, _F, _D]
+        L.laps_closure_filter.argtypes = [vp, ctypes.c_int, _D]
         L.laps_mlp_forward.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F]
         L.laps_mlp_grad.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, _F, _F]
         L.laps_mlp_train.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, ctypes.c_int, ctypes.c_int,
@@ -112,18 +114,31 @@ def _sptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
-def mlp_forward(X, params, device=0):
-    """Closure network on samples X (n, 28) -> (n, 15)."""
+def _samples(X, T=None):
+    """the network's samples as contiguous FP32 arrays, shapes checked: X (n, 28), T (n, 18)"""
     X = _f32(X)
+    if X.ndim != 2 or X.shape[1] != MLP_NIN:
+        raise LapsError(f"network inputs must be (n, {MLP_NIN}), got {X.shape}")
+    if T is None:
+        return X
+    T = _f32(T)
+    if T.shape != (X.shape[0], MLP_NOUT):
+        raise LapsError(f"network targets must be ({X.shape[0]}, {MLP_NOUT}), got {T.shape}")
+    return X, T
+
+
+def mlp_forward(X, params, device=0):
+    """Closure network on samples X (n, 28) -> (n, 18)."""
+    X = _samples(X)
     P = _f32(params, (MLP_NPARAM,))
-    Y = np.zeros((X.shape[0], 15), np.float32)
+    Y = np.zeros((X.shape[0], MLP_NOUT), np.float32)
     _check(lib().laps_mlp_forward(device, X.shape[0], _sptr(X), _sptr(P), _sptr(Y)))
     return Y
 
 
 def mlp_grad(X, T, params, device=0):
     """Gradient of the mean squared error over all samples, and the loss."""
-    X, T = _f32(X), _f32(T)
+    X, T = _samples(X, T)
     P = _f32(params, (MLP_NPARAM,))
     G = np.zeros(MLP_NPARAM, np.float32)
     loss = ctypes.c_float()
@@ -134,7 +149,7 @@ def mlp_grad(X, T, params, device=0):
 def mlp_train(X, T, params, steps, batch, lr, device=0):
     """Adam steps over consecutive batches of the (already shuffled) samples; returns the new
     parameters and the loss of every batch."""
-    X, T = _f32(X), _f32(T)
+    X, T = _samples(X, T)
     P = _f32(params, (MLP_NPARAM,)).copy()
     hist = np.zeros(max(steps, 1), np.float32)
     _check(lib().laps_mlp_train(device, X.shape[0], _sptr(X), _sptr(T), _sptr(P), int(steps), int(batch),
@@ -286,11 +301,18 @@ class Laps:
         return out
 
     def closure_eval(self):
-        """Normalised network outputs (nx*ny*nz, 15) and the closure RHS term (nxh, ny, nz, 6)."""
-        out = np.zeros((self.nx * self.ny * self.nz, 15), np.float32)
-        rhs = self._spec(6)
+        """Normalised network outputs (nx*ny*nz, 18) and the closure RHS term (nxh, ny, nz, 7)."""
+        out = np.zeros((self.nx * self.ny * self.nz, 18), np.float32)
+        rhs = self._spec(7)
         _check(lib().laps_closure_eval(self._h, _sptr(out), _fptr(rhs)))
         return out, rhs
+
+    def closure_filter(self, r):
+        """Box filter (r^3 cells) of the state and of its flux products, on the device
+        (laps_closure_filter): (nx/r, ny/r, nz/r, 26) FP64"""
+        out = np.zeros((self.nx // r, self.ny // r, self.nz // r, 26), order="F")
+        _check(lib().laps_closure_filter(self._h, int(r), _fptr(out)))
+        return out
 
     def fft_r2c(self, real):
         real = np.asfortranarray(real, dtype=np.float64)

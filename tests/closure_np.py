@@ -5,7 +5,7 @@ import numpy as np
 
 from .helpers import wave_numbers
 
-NIN, NH, NOUT = 28, 64, 15
+NIN, NH, NOUT = 28, 64, 18
 PAIRS = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
 
 
@@ -58,33 +58,35 @@ def features(uu, prim, L):
 
 
 def terms(Yn, ystd, gamma):
-    """(N, 9): momentum-flux correction xx, xy, xz, yy, yz, zz and the EMF correction."""
+    """(N, 12): momentum-flux correction xx, xy, xz, yy, yz, zz, the EMF correction and the
+    energy-flux correction."""
     o = np.asarray(Yn, np.float64) * np.asarray(ystd, np.float64)
     trR = o[:, 0] + o[:, 3] + o[:, 5]
     trM = o[:, 6] + o[:, 9] + o[:, 11]
     q = -0.5 * (gamma - 1.0) * (trR + trM) + 0.5 * trM
-    C = np.empty((o.shape[0], 9))
+    C = np.empty((o.shape[0], 12))
     for c in range(6):
         C[:, c] = o[:, c] - o[:, 6 + c] + (q if c in (0, 3, 5) else 0.0)
-    C[:, 6:9] = o[:, 12:15]
+    C[:, 6:12] = o[:, 12:18]
     return C
 
 
 def rhs(C, shape, L):
-    """Spectral RHS term (nx/2+1, ny, nz, 6): -i k.tau for momentum and -i k x E for B,
-    with spectra normalised by 1/N like the solver's."""
+    """Spectral RHS term (nx/2+1, ny, nz, 7): -i k.tau for momentum, -i k x E for B and
+    -i k.F for the energy, with spectra normalised by 1/N like the solver's."""
     nx, ny, nz = shape
     N = nx * ny * nz
-    F = [np.fft.rfftn(C[:, c].reshape(shape, order="F"), axes=(2, 1, 0)) / N for c in range(9)]
+    F = [np.fft.rfftn(C[:, c].reshape(shape, order="F"), axes=(2, 1, 0)) / N for c in range(12)]
     Kx, Ky, Kz = np.meshgrid(wave_numbers(nx, L[0])[: nx // 2 + 1], wave_numbers(ny, L[1]),
                              wave_numbers(nz, L[2]), indexing="ij")
     K = (Kx, Ky, Kz)
     idx = ((0, 1, 2), (1, 3, 4), (2, 4, 5))
-    out = np.empty(F[0].shape + (6,), complex)
+    out = np.empty(F[0].shape + (7,), complex)
     for a in range(3):
         out[..., a] = -1j * sum(K[b] * F[idx[a][b]] for b in range(3))
     E = F[6:9]
     for a in range(3):
         b, c = (a + 1) % 3, (a + 2) % 3
         out[..., 3 + a] = -1j * (K[b] * E[c] - K[c] * E[b])
+    out[..., 6] = -1j * sum(K[b] * F[9 + b] for b in range(3))
     return out

@@ -4,6 +4,7 @@ block variance, the tensor symmetry)."""
 import math
 
 import numpy as np
+import pytest
 
 from laps_amd import closure
 from tests import closure_np as ref
@@ -28,10 +29,10 @@ def test_numpy_gradient_matches_finite_differences():
 
 
 def test_parameter_layout():
-    assert closure.init_params(0).size == 6991
-    W1, b1, W2, b2, W3, b3 = ref.unpack(np.arange(6991.0))
-    assert W1.shape == (28, 64) and W2.shape == (64, 64) and W3.shape == (64, 15)
-    assert b3[-1] == 6990.0
+    assert closure.init_params(0).size == 7186
+    W1, b1, W2, b2, W3, b3 = ref.unpack(np.arange(7186.0))
+    assert W1.shape == (28, 64) and W2.shape == (64, 64) and W3.shape == (64, 18)
+    assert b3[-1] == 7185.0
 
 
 def test_constant_state_has_no_subgrid_terms():
@@ -43,7 +44,9 @@ def test_constant_state_has_no_subgrid_terms():
     uu[..., 7] = 4.0
     prim = np.zeros((16, 16, 16, 4), order="F")
     prim[..., 0:3] = uu[..., 1:4] / 1.3
-    prim[..., 3] = 1.0
+    # the pressure of the state (update_uu_prim_from_uu, mhdrhs.f90:282-294)
+    prim[..., 3] = (p.adiabatic_index - 1) * (4.0 - 0.5 * (uu[..., 1:4] * prim[..., 0:3]).sum(-1)
+                                              - 0.5 * (uu[..., 4:7] ** 2).sum(-1))
     uu_c, prim_c, T = closure.filtered_sample(uu, prim, 2, p.adiabatic_index, 0.05, (p.Lx, p.Ly, p.Lz))
     assert uu_c.shape == (8, 8, 8, 8)
     assert np.abs(T).max() < 1e-14
@@ -82,3 +85,39 @@ def test_numpy_features_of_a_single_mode():
     expect = (math.sin(2 * dx) / dx * np.cos(2 * x))[:, None, None] * np.ones(n)
     np.testing.assert_allclose(X[:, 7 + 3 * 4], expect.reshape(-1, order="F"), atol=1e-12)
     assert np.abs(X[:, 8 + 3 * 4]).max() == 0.0
+
+
+def test_energy_flux_target_closed_form():
+    # rho = 1, B = 0, p = p0 uniform, u = (U(x,y,z), 0, 0): e = p0/(g-1) + U^2/2, the energy flux
+    # (mhdrhs.f90:87-89) is (e + p) U; the coarse pressure carries the filtered kinetic energy
+    # of the subgrid motion, p_c = p0 + (g-1)/2 (bar(U^2) - bar(U)^2), so the subgrid flux is
+    # bar(U^3)/2 - bar(U^2) bar(U)/2 - (g-1)/2 (bar(U^2) - bar(U)^2) bar(U), and zero across x
+    p = params(16)
+    g, p0 = p.adiabatic_index, 0.7
+    rng = np.random.default_rng(5)
+    U = rng.standard_normal((16, 16, 16)) * 0.3
+    uu = np.zeros((16, 16, 16, 8), order="F")
+    uu[..., 0] = 1.0
+    uu[..., 1] = U
+    uu[..., 7] = p0 / (g - 1) + 0.5 * U * U
+    prim = np.zeros((16, 16, 16, 4), order="F")
+    prim[..., 0] = U
+    prim[..., 3] = p0
+    _, _, T = closure.filtered_sample(uu, prim, 2, g, 0.0, (p.Lx, p.Ly, p.Lz))
+
+    def bar(f):
+        return f.reshape(2, 8, 2, 8, 2, 8, order="F").mean(axis=(0, 2, 4)).reshape(-1, order="F")
+
+    ub, u2, u3 = bar(U), bar(U * U), bar(U ** 3)
+    want = 0.5 * u3 - 0.5 * u2 * ub - 0.5 * (g - 1) * (u2 - ub * ub) * ub
+    np.testing.assert_allclose(T[:, 15], want, atol=1e-14)
+    assert np.abs(T[:, 16:18]).max() < 1e-15
+
+
+def test_network_sample_shapes_are_checked():
+    from laps_amd import runtime
+    X = np.zeros((10, 28), np.float32)
+    with pytest.raises(runtime.LapsError, match="targets"):
+        runtime.mlp_grad(X, np.zeros((10, 15), np.float32), np.zeros(runtime.MLP_NPARAM, np.float32))
+    with pytest.raises(runtime.LapsError, match="inputs"):
+        runtime.mlp_forward(np.zeros((10, 27), np.float32), np.zeros(runtime.MLP_NPARAM, np.float32))

@@ -27,7 +27,7 @@ def random_params(seed):
 def test_network_forward_and_gradient_vs_numpy(rt):
     rng = np.random.default_rng(1)
     X = rng.standard_normal((3000, 28)).astype(np.float32)
-    T = rng.standard_normal((3000, 15)).astype(np.float32)
+    T = rng.standard_normal((3000, 18)).astype(np.float32)
     P = random_params(2)
     Y = rt.mlp_forward(X, P)
     Yr = ref.forward(X.astype(np.float64), P)[0]
@@ -74,7 +74,7 @@ def test_closure_outputs_and_rhs_vs_numpy(rt):
     P = random_params(8)
     xmean = rng.standard_normal(28).astype(np.float32) * 0.1
     xstd = (0.5 + rng.random(28)).astype(np.float32)
-    ystd = (1e-3 * (0.5 + rng.random(15))).astype(np.float32)
+    ystd = (1e-3 * (0.5 + rng.random(18))).astype(np.float32)
     vec = closure.closure_vector(P, xmean, xstd, ystd)
     with rt.Laps(p) as g:
         g.set_state(uu, pr)
@@ -95,7 +95,7 @@ def test_zero_network_leaves_the_step_unchanged(rt):
     p = params(32, if_hall=1, ion_inertial_length=0.05)
     uu, pr = seeded_state(p, seed=9, kmax=6, du=0.2, db=0.2)
     vec = closure.closure_vector(np.zeros(rt.MLP_NPARAM, np.float32), np.zeros(28, np.float32),
-                                 np.ones(28, np.float32), np.ones(15, np.float32))
+                                 np.ones(28, np.float32), np.ones(18, np.float32))
     with rt.Laps(p) as a, rt.Laps(p) as b:
         for g in (a, b):
             g.set_state(uu, pr)
@@ -107,3 +107,31 @@ def test_zero_network_leaves_the_step_unchanged(rt):
         a.step(1)
         b.step(1)
         assert np.array_equal(a.spectral(), b.spectral())
+
+
+def test_device_filter_vs_numpy(rt):
+    """laps_closure_filter (box filter of the state and its flux products, J from the library's
+    own transforms) against the same definitions in numpy (laps_amd.closure.filtered_sample's
+    products, J = curl B by scipy; the state has no mode near Nyquist, where the two transforms'
+    derivative conventions differ)."""
+    p = params((32, 16, 64), Ly=3.0, Lz=5.0, if_hall=1, ion_inertial_length=0.05)
+    uu, pr = seeded_state(p, seed=12, kmax=4, du=0.2, db=0.2)
+    L = (p.Lx, p.Ly, p.Lz)
+    r = 2
+    with rt.Laps(p) as g:
+        g.set_state(uu, pr)
+        g.step(2)
+        f = g.closure_filter(r)
+        u2, q2 = g.state()
+    rho, m, B = u2[..., 0], u2[..., 1:4], u2[..., 4:7]
+    u = q2[..., 0:3]
+    E = closure.emf(rho, u, B, closure.curl(B, L), 0.05)
+    F = closure.energy_flux(u2[..., 7], u, B, q2[..., 3])
+    cols = [u2[..., c] for c in range(8)]
+    cols += [m[..., i] * u[..., j] for i, j in closure.PAIRS]
+    cols += [B[..., i] * B[..., j] for i, j in closure.PAIRS]
+    cols += [E[..., a] for a in range(3)] + [F[..., a] for a in range(3)]
+    for q, c in enumerate(cols):
+        want = closure.box_filter(c, r)
+        err = np.abs(f[..., q] - want).max() / max(np.abs(want).max(), 1e-30)
+        assert err < 1e-11, (q, err)

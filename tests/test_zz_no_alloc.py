@@ -59,7 +59,7 @@ def test_noalloc_grad_equals_allocating_grad():
         for n in (3000, 700):   # the second call is smaller: nothing may depend on the first's contents
             print(f"noalloc gradient: n = {n}", flush=True)
             X = rng.standard_normal((n, 28)).astype(np.float32)
-            T = rng.standard_normal((n, 15)).astype(np.float32)
+            T = rng.standard_normal((n, runtime.MLP_NOUT)).astype(np.float32)
             g_ref, l_ref = runtime.mlp_grad(X, T, P)
             nan_g = np.full(runtime.MLP_NPARAM, np.nan, np.float32)
             nan_l = np.full(1, np.nan, np.float32)
