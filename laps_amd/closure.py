@@ -206,6 +206,13 @@ def initial_state(p_fine, seed, amp, ic):
     return cons_from_prim(prim, p_fine.adiabatic_index)
 
 
+def check_finite(what, *arrays):
+    for a in arrays:
+        if not np.all(np.isfinite(a)):
+            raise FloatingPointError(f"closure experiment: non-finite values in the {what} data "
+                                     f"(the fixed dt or the viscosity does not hold this run)")
+
+
 def fine_samples(p_fine, p_coarse, state, r, nsteps, every, dt):
     """Training samples of a fine run at a fixed dt, every `every` steps: (step, uu_c, prim_c,
     X, T) with the filtered state, its raw features and the subgrid targets, all formed on the
@@ -220,7 +227,7 @@ def fine_samples(p_fine, p_coarse, state, r, nsteps, every, dt):
 
 
 def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, every=32, amp=0.4,
-               di=0.05, visc=5e-3, train_steps=4000, ic="kolmogorov", log=print):
+               di=0.05, visc=5e-3, train_steps=4000, ic="kolmogorov", dt_frac=0.8, log=print):
     """Trains the closure on box-filtered fine runs and compares coarse runs with and
     without it against the filtered fine run of another seed: energy spectra per shell up
     to the coarse grid's dealiased range, and the cost of a coarse step with the closure
@@ -233,19 +240,21 @@ def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, 
                         if_hall=1 if di else 0, ion_inertial_length=di)
     pc = coarse_params(pf, r)
     nc = pc.nx
-    # the fine dt: 0.8 of the CFL dt of the first training state, kept for every run
+    # the fine dt: dt_frac times the CFL dt of the first training state, kept for every run
     t0 = time.perf_counter()
     state0 = initial_state(pf, train_seeds[0], amp, ic)
     with Laps(pf) as g:
         g.set_state(*state0)
-        dt = 0.8 * g.cfl()[1]
+        dt = dt_frac * g.cfl()[1]
     X, T = [], []
     for k, sd in enumerate(train_seeds):
         st = state0 if k == 0 else initial_state(pf, sd, amp, ic)
         for s, uc, qc, x, t in fine_samples(pf, pc, st, r, nsteps, every, dt):
+            check_finite(f"seed {sd}, fine step {s}", uc, x, t)
             X.append(x)
             T.append(t.astype(np.float32))
-            log(f"closure: seed {sd}, fine step {s}: {x.shape[0]} samples ({time.perf_counter() - t0:.1f} s)")
+            log(f"closure: seed {sd}, fine step {s}: {x.shape[0]} samples, filtered energy "
+                f"{0.5 * float(np.mean(uc[..., 1:7] ** 2)):.5e} ({time.perf_counter() - t0:.1f} s)")
         del st
     del state0
     X = np.concatenate(X)
@@ -261,6 +270,7 @@ def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, 
     log(f"closure: evaluation run, seed {test_seed}, {nsteps} fine steps")
     snaps = []
     for snap in fine_samples(pf, pc, initial_state(pf, test_seed, amp, ic), r, nsteps, max(1, nsteps // 4), dt):
+        check_finite(f"evaluation seed {test_seed}, fine step {snap[0]}", snap[1])
         snaps.append(snap[:3])
         log(f"closure: evaluation run at fine step {snap[0]} ({time.perf_counter() - t0:.1f} s)")
     (_, uc0, qc0), (_, uc1, qc1) = snaps[0], snaps[-1]
@@ -288,8 +298,10 @@ def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, 
                 g.closure_set(vec)
             g.step(nsteps // r)
             e = shell_spectrum(g.spectral(), nc, kmax)
+        check_finite(f"coarse {tag} run", e)
         dev = np.abs(e[1:] - e_ref[1:]) / e_ref[1:]
-        within = int(np.argmax(dev > 0.05)) if np.any(dev > 0.05) else len(dev)
+        off = ~(dev <= 0.05)
+        within = int(np.argmax(off)) if np.any(off) else len(dev)
         res[tag] = dict(max_shell_dev=float(dev.max()), mean_shell_dev=float(dev.mean()),
                         shells_within_5pct=within, shell_dev=[round(float(x), 4) for x in dev], s_per_step=tc)
         log(f"closure: coarse {nc}^3 {tag}: max shell deviation {dev.max():.4f}, mean {dev.mean():.4f}, "
@@ -311,6 +323,7 @@ def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, 
                          f"nu=eta={visc}, {icname} {amp}; trained on seeds {list(train_seeds)}, "
                          f"evaluated on seed {test_seed} after {nsteps} fine steps ({nsteps // r} coarse)",
                 samples=int(sum(1 for _ in range(0, nsteps + 1, every)) * len(train_seeds) * nc ** 3),
+                dt_fine=dt, dt_coarse=r * dt,
                 train_steps=train_steps, loss_first=float(hist[:50].mean()), loss_last=float(hist[-50:].mean()),
                 shells=f"1..{kmax}", plain=res["plain"], closure=res["closure"],
                 cost_ratio_fine_over_coarse_closure=round(float(cost), 2))
@@ -330,7 +343,8 @@ if __name__ == "__main__":
     ap.add_argument("--amp", type=float, default=0.4)
     ap.add_argument("--di", type=float, default=0.05)
     ap.add_argument("--ic", choices=["kolmogorov", "alfvenic"], default="kolmogorov")
+    ap.add_argument("--dt-frac", type=float, default=0.8, help="fixed fine dt as a fraction of the first CFL dt")
     a = ap.parse_args()
     print(json.dumps(experiment(n_fine=a.n_fine, r=a.ratio, nsteps=a.steps, every=a.every, visc=a.visc,
-                                train_steps=a.train_steps, amp=a.amp, di=a.di, ic=a.ic,
+                                train_steps=a.train_steps, amp=a.amp, di=a.di, ic=a.ic, dt_frac=a.dt_frac,
                                 log=lambda m: print(m, file=sys.stderr, flush=True))))

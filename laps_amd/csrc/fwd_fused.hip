@@ -39,6 +39,9 @@ struct FwdCfg {
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
+    // waves per SIMD the register allocation must allow: the FP32 y pass at 256 (512-thread
+    // blocks, 66 KB of LDS) fits two blocks per CU only at <= 128 VGPRs
+    static constexpr int MINW = (sizeof(T) == 4 && N == 256 && !ZP) ? 4 : 1;
     static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
@@ -58,7 +61,7 @@ __device__ __forceinline__ cpx<T> mul_i(T k, cpx<T> z) { return {-k * z.y, k * z
 
 // ---------------------------------------------------------------- y pass
 template <typename T, int N>
-__global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const cpx<T>* __restrict__ in,
+__global__ __launch_bounds__((FwdCfg<T, N, false>::NT), (FwdCfg<T, N, false>::MINW)) void k_yfwd_group(const cpx<T>* __restrict__ in,
                                                                  cpx<T>* __restrict__ out, YGArgs a,
                                                                  const cpx<T>* __restrict__ tw,
                                                                  const double* __restrict__ wxt,
@@ -66,15 +69,18 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
                                                                  const Ctl* __restrict__ ctl, int ntile, int nz) {
     using C = FwdCfg<T, N, false>;
     using I = typename C::I;
+    // FP32: the derivative factors come from LDS tables of their y-dependent parts (below)
+    constexpr bool TAB = sizeof(T) == 4;
     __shared__ cpx<T> sh[C::TX * C::LS];
     __shared__ cpx<T> stw[N];
-    __shared__ double swy[N];
+    __shared__ double swy[TAB ? 1 : N];
+    __shared__ T sax[TAB ? N : 1], say[TAB ? N : 1];
     LdsTable<cpx<T>, N, C::NT> twt;
     LdsTable<double, N, C::NT> wyt;
     twt.fetch(tw);
     wyt.fetch(wyf);
     twt.put(stw);  // visible after the barrier that opens each transform
-    wyt.put(swy);
+    if constexpr (!TAB) wyt.put(swy);
     const int l = threadIdx.x % C::TX, j = threadIdx.x / C::TX;
     in += blockIdx.y * a.in_bx;    // batched boxes: one per grid row
     out += blockIdx.y * a.out_bx;
@@ -96,6 +102,37 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
     const int kx = valid ? col : 0;
     const double radius = ctl->radius;
     const double wx = wxt[kx];
+    // derivative factors kx', ky' of output position p of this column (mhdrhs.f90:192-204).
+    // FP64: the reference's expressions per mode (kprime_xy). FP32: kx' = cx + ax[p] and
+    // ky' = cy + ay[p], the y-dependent parts tabulated in LDS once per block (the pass
+    // rounds the factors to FP32 either way), so no factor is held in registers across
+    // transforms and no FP64 quotient is formed per mode.
+    T cx = T(0), cy = T(0);
+    if constexpr (TAB) {
+        const double rr = pp.radius0 / radius;
+#pragma unroll
+        for (int i = 0; i < decltype(wyt)::PER; ++i) {
+            const int t = (int)threadIdx.x + i * C::NT;
+            if (t < N) {
+                const double wy = wyt.r[i];
+                sax[t] = pp.corot_k ? T(wy * pp.sin_c) : T(0);
+                say[t] = pp.corot_k ? T(wy * pp.cos_c * rr) : T(wy * rr);
+            }
+        }
+        cx = pp.corot_k ? T(wx * pp.cos_c) : T(wx);
+        cy = pp.corot_k ? T(-wx * pp.sin_c * rr) : T(0);
+    }
+    auto kxy = [&](int p, T& kxv, T& kyv) {
+        if constexpr (TAB) {
+            kxv = cx + sax[p];
+            kyv = cy + say[p];
+        } else {
+            double kxp, kyp;
+            kprime_xy(wx, swy[p], pp, radius, kxp, kyp);
+            kxv = T(kxp);
+            kyv = T(kyp);
+        }
+    };
     const int ylim = a.ylim ? a.ylim[kx] : N;
     cpx<T>* line = sh + l * C::LS;
     auto lds = [&](int p) { return line + p; };
@@ -122,35 +159,43 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
         // each off-diagonal component is transformed once and used by both rows)
         auto neg = [](cpx<T> z) { return cpx<T>{-z.x, -z.y}; };
         const bool h = fft_field(G.in[0], va);                                      // xx
-        T kxr[I::RLAST], kyr[I::RLAST];  // swy is visible after the transform's first barrier
+        // the tables are visible after the transform's first barrier; FP64 keeps the
+        // factors in registers (4 per work-item), FP32 reads them from the tables at each use
+        constexpr int NKC = TAB ? 1 : I::RLAST;
+        T kxc[NKC], kyc[NKC];
+        if constexpr (!TAB) {
 #pragma unroll
-        for (int r = 0; r < I::RLAST; ++r) {
-            double kxp, kyp;
-            kprime_xy(wx, swy[h ? j + r * I::BLAST : 0], pp, radius, kxp, kyp);
-            kxr[r] = T(kxp);
-            kyr[r] = T(kyp);
+            for (int r = 0; r < I::RLAST; ++r) kxy(h ? j + r * I::BLAST : 0, kxc[r], kyc[r]);
         }
+        auto kxr = [&](int r) -> T {
+            if constexpr (TAB) return cx + sax[h ? j + r * I::BLAST : 0];
+            else return kxc[r];
+        };
+        auto kyr = [&](int r) -> T {
+            if constexpr (TAB) return cy + say[h ? j + r * I::BLAST : 0];
+            else return kyc[r];
+        };
 #pragma unroll
-        for (int r = 0; r < I::RLAST; ++r) va[r] = mul_i(kxr[r], va[r]);
+        for (int r = 0; r < I::RLAST; ++r) va[r] = mul_i(kxr(r), va[r]);
         fft_field(G.in[1], vb);                                                     // xy
         if (h) {
 #pragma unroll
             for (int r = 0; r < I::RLAST; ++r) {
-                store(G.out[0], j + r * I::BLAST, neg(va[r] + mul_i(kyr[r], vb[r])));
-                vb[r] = mul_i(kxr[r], vb[r]);                                       // y row, x part
+                store(G.out[0], j + r * I::BLAST, neg(va[r] + mul_i(kyr(r), vb[r])));
+                vb[r] = mul_i(kxr(r), vb[r]);                                       // y row, x part
             }
         }
         fft_field(G.in[3], va);                                                     // yy
         if (h) {
 #pragma unroll
-            for (int r = 0; r < I::RLAST; ++r) store(G.out[2], j + r * I::BLAST, neg(vb[r] + mul_i(kyr[r], va[r])));
+            for (int r = 0; r < I::RLAST; ++r) store(G.out[2], j + r * I::BLAST, neg(vb[r] + mul_i(kyr(r), va[r])));
         }
         fft_field(G.in[2], va);                                                     // xz
         if (h) {
 #pragma unroll
             for (int r = 0; r < I::RLAST; ++r) {
                 store(G.out[1], j + r * I::BLAST, va[r]);
-                va[r] = mul_i(kxr[r], va[r]);                                       // z row, x part
+                va[r] = mul_i(kxr(r), va[r]);                                       // z row, x part
             }
         }
         fft_field(G.in[4], vb);                                                     // yz
@@ -158,7 +203,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
 #pragma unroll
             for (int r = 0; r < I::RLAST; ++r) {
                 store(G.out[3], j + r * I::BLAST, vb[r]);
-                store(G.out[4], j + r * I::BLAST, neg(va[r] + mul_i(kyr[r], vb[r])));
+                store(G.out[4], j + r * I::BLAST, neg(va[r] + mul_i(kyr(r), vb[r])));
             }
         }
         fft_field(G.in[5], va);                                                     // zz
@@ -185,9 +230,8 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
 #pragma unroll
         for (int r = 0; r < I::RLAST; ++r) {
             const int p = j + r * I::BLAST;
-            double kxp, kyp;
-            kprime_xy(wx, swy[p], pp, radius, kxp, kyp);
-            const T kx_ = T(kxp), ky_ = T(kyp);
+            T kx_, ky_;
+            kxy(p, kx_, ky_);
             if (G.kind == 0) {
                 const cpx<T> gsum = mul_i(kx_, va[r]) + mul_i(ky_, vb[r]);
                 store(G.out[0], p, cpx<T>{-gsum.x, -gsum.y});                  // -(i kx' Ya + i ky' Yb)
@@ -205,11 +249,11 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
             if (G.kind == 0) {
                 store(G.out[1], p, va[r]);                                       // Y(F_c)
             } else {
-                double kxp, kyp;
-                kprime_xy(wx, swy[p], pp, radius, kxp, kyp);
-                const cpx<T> t5 = mul_i(T(kyp), va[r]);
+                T kx_, ky_;
+                kxy(p, kx_, ky_);
+                const cpx<T> t5 = mul_i(ky_, va[r]);
                 store(G.out[2], p, cpx<T>{-t5.x, -t5.y});                        // P5 = -i ky' Y(E_z)
-                store(G.out[3], p, mul_i(T(kxp), va[r]));                        // P6 =  i kx' Y(E_z)
+                store(G.out[3], p, mul_i(kx_, va[r]));                           // P6 =  i kx' Y(E_z)
             }
         }
     }
@@ -218,7 +262,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT)) void k_yfwd_group(const 
 // ---------------------------------------------------------------- z pass + RK3
 // Output variable v is  sum_t  m_t * Z(D[f_t])  with m_t in {1, -i kz', +i kz'}.
 template <typename T, int N, int OPT>
-__global__ __launch_bounds__((FwdCfg<T, N, true>::NT)) void k_zfwd_rhs(const cpx<T>* __restrict__ D,
+__global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW)) void k_zfwd_rhs(const cpx<T>* __restrict__ D,
                                                                cpx<double>* __restrict__ U,
                                                                cpx<T>* __restrict__ Fp, ZArgs a,
                                                                const cpx<T>* __restrict__ tw, BoxTables tb,
