@@ -42,7 +42,7 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
     in += blockIdx.y * a.in_bx;    // batched boxes: one per grid row
     out += blockIdx.y * a.out_bx;
     const cpx<double>* ph = a.ph;
-    if (a.nch == 2) {
+    if (a.nch == 2 && !a.dual) {
         int ch;
         if (a.pair) {
             // blocks b and b + 8 run the same slab of chains 0 and 1: work-groups are dealt
@@ -104,6 +104,39 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
         if (valid && s >= 0 && !dead) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
     };
     cpx<T>* const stp = stw;
+    if (a.dual) {
+        // both chains: chain 0 from the input as loaded, chain 1 from the same values times the
+        // grid shift, so the input is read and waited for once per slab
+        using I = PlanInfo<N, P>;
+        cpx<T> v[I::RMAX], w[I::RMAX];
+        if (j < I::B0) {
+#pragma unroll
+            for (int r = 0; r < P::R0; ++r) {
+                const int p = j + r * I::B0;
+                const int s = pos_to_stored(p, N, in_cnt);
+                cpx<T> x{T(0), T(0)};
+                if (valid && s >= 0 && !dead) {
+                    const cpx<TI> q = ib[s * in_ls];
+                    x = cpx<T>{T(q.x), T(q.y)};
+                }
+                v[r] = x;
+                const cpx<double> sh1 = cmul(ph[p], wcb);
+                w[r] = cmul(x, cpx<T>{T(sh1.x), T(sh1.y)});
+            }
+        }
+        twt.put(stp);
+        if (line_fft_stages<N, DIR, T, P>(j, stw, lds, v)) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) store(j + r * I::BLAST, v[r]);
+        }
+        __syncthreads();  // every read of the LDS line before the second transform writes it
+        ob += a.out_ch;
+        if (line_fft_stages<N, DIR, T, P>(j, stw, lds, w)) {
+#pragma unroll
+            for (int r = 0; r < I::RLAST; ++r) store(j + r * I::BLAST, w[r]);
+        }
+        return;
+    }
     line_fft<N, DIR, T, P>(j, stw, load, lds, store, [twt, stp]() { twt.put(stp); });
 }
 
@@ -212,7 +245,7 @@ static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs
     const int ntile = (a.ncol + C::TX - 1) / C::TX;
     const long long nrest = (long long)ntile * a.nb1 * a.nb2;
     if (nrest <= 0) return hipSuccess;
-    const long long nblk = a.nch == 2 ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest;
+    const long long nblk = (a.nch == 2 && !a.dual) ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest;
     hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P, W>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0,
                        s, in, out, a, tw, scale, ntile, nrest);
     return hipGetLastError();

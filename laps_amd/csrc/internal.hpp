@@ -33,6 +33,9 @@ struct StridedArgs {
     int nch;
     long long in_ch, out_ch;
     bool pair;
+    // dual (nch = 2, both chains read the same input, chain 1 shifted): one work-group runs
+    // both chains of its slab, loading the input once
+    bool dual;
     // optional rows of the dealiasing mask (mask 1: 2/3 sphere, 3: phase-shift ellipsoid) for
     // a kept-box state that is zero outside the kept set: a row (kx = col, my) with no kept
     // mode even at mz = 0 is all zero. row_role 1 (z inverse, slab b1 = stored ky row of

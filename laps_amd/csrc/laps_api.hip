@@ -418,6 +418,7 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
     z.nb1 = b.CY; z.in_b1 = RS; z.out_b1 = RS;
     z.nb2 = nf; z.in_b2 = fs; z.out_b2 = NZ * b.CY * RS;
     z.nch = nch; z.out_ch = a_ch; z.pair = true;
+    z.dual = nch == 2;  // the shifted chain from the plain chain's loads, in the same work-group
     z.nbox = nbox; z.in_bx = src_bx; z.out_bx = a_bx;
     // masked: src is the kept-box state after a masked substage (zero outside the kept set);
     // the mask rows with no kept mode are skipped by the z inverse and read as zero by the y
@@ -651,10 +652,18 @@ int substage(laps_ctx* c, int irk) {
 template <typename T>
 int autotune(laps_ctx* c) {
     // LAPS_FFT_PLAN = 0 or 1 fixes both plans (default tile width) instead (reproducible runs:
-    // the timed choice can differ between contexts, and with it the rounding of the transforms)
+    // the timed choice can differ between contexts, and with it the rounding of the transforms);
+    // "z,y" (two variants 0-3) fixes the z and y inverse variants separately
     if (const char* fixed = std::getenv("LAPS_FFT_PLAN")) {
         if ((fixed[0] == '0' || fixed[0] == '1') && fixed[1] == 0) {
             c->plan_zinv = c->plan_yinv = fixed[0] - '0';
+            c->tuned = true;
+            return LAPS_OK;
+        }
+        if (fixed[0] >= '0' && fixed[0] <= '3' && fixed[1] == ',' && fixed[2] >= '0' && fixed[2] <= '3' &&
+            fixed[3] == 0) {
+            c->plan_zinv = fixed[0] - '0';
+            c->plan_yinv = fixed[2] - '0';
             c->tuned = true;
             return LAPS_OK;
         }
@@ -712,27 +721,48 @@ int autotune(laps_ctx* c) {
             best_y[v] = (rep == 1 || ms_y < best_y[v]) ? ms_y : best_y[v];
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     // a variant other than the default only when clearly faster (3 %), so repeated runs pick
     // the same code; the z pass is compared on its own time (z + y with the y variant held at 0,
     // less that y time)
     int vz = 0, vy = 0;
-    float tz = best_zy[0] - best_y[0], ty = best_y[0];
+    float tz = best_zy[0] - best_y[0];
     for (int v = 1; v < NV; ++v) {
         const float z = best_zy[v] - best_y[0];
         if (z < 0.97f * tz) { tz = z; vz = v; }
-        if (best_y[v] < 0.97f * ty) { ty = best_y[v]; vy = v; }
     }
+    // the y pass is compared where a step runs it: right after the chosen z pass (its time
+    // alone, on an input written long before, has ranked the variants differently)
+    float best_zyv[NV] = {0, 0, 0, 0};
+    for (int rep = 0; rep < 4; ++rep) {
+        for (int v = 0; v < NV; ++v) {
+            float ms = 0;
+            const int rc = time_it([&] {
+                return inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, vz, v,
+                                          nch == 2, nch, c->p1, c->p2, true);
+            }, ms);
+            if (rc) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return rc;
+            }
+            if (rep == 0) continue;  // warm-up
+            best_zyv[v] = (rep == 1 || ms < best_zyv[v]) ? ms : best_zyv[v];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    float ty = best_zyv[0];
+    for (int v = 1; v < NV; ++v)
+        if (best_zyv[v] < 0.97f * ty) { ty = best_zyv[v]; vy = v; }
     c->plan_zinv = vz;
     c->plan_yinv = vy;
     c->tuned = true;
     if (std::getenv("LAPS_TUNE_LOG")) {
         std::fprintf(stderr, "[laps] autotune %dx%dx%d %s: z inverse", c->nx, c->ny, c->nz, c->mixed ? "mixed" : "fp64");
         for (int v = 0; v < NV; ++v) std::fprintf(stderr, " %.3f", best_zy[v] - best_y[0]);
-        std::fprintf(stderr, " ms -> variant %d; y inverse", vz);
-        for (int v = 0; v < NV; ++v) std::fprintf(stderr, " %.3f", best_y[v]);
-        std::fprintf(stderr, " ms -> variant %d\n", vy);
+        std::fprintf(stderr, " ms -> variant %d; z + y inverse", vz);
+        for (int v = 0; v < NV; ++v) std::fprintf(stderr, " %.3f", best_zyv[v]);
+        std::fprintf(stderr, " ms -> y variant %d\n", vy);
     }
     return LAPS_OK;
 }
