@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace laps {
 
 template <typename T>
@@ -182,14 +184,29 @@ template <> struct PlanLow<256> { static constexpr int NST = 3, R0 = 8, R1 = 8, 
 template <> struct PlanLow<512> { static constexpr int NST = 3, R0 = 8, R1 = 8, R2 = 8; };
 template <> struct PlanLow<96>  { static constexpr int NST = 3, R0 = 4, R1 = 4, R2 = 6; };
 
+// PlanLowW: the radices of PlanLow<256> (8·8·4), with two radix-4 butterflies per work-item in
+// the last stage, so that 32 work-items per line are busy in every stage (PlanLow's last stage
+// needs 64 and leaves half of them idle in the first two); 8 outputs per work-item.
+template <int N>
+struct PlanLowW;
+template <> struct PlanLowW<256> { static constexpr int NST = 3, R0 = 8, R1 = 8, R2 = 4, S2 = 2; };
+
+// last-stage butterflies per work-item (1 unless the plan says otherwise)
+template <class P, class = void>
+struct LastSplit { static constexpr int S = 1; };
+template <class P>
+struct LastSplit<P, std::void_t<decltype(P::S2)>> { static constexpr int S = P::S2; };
+
 template <int N, class P = Plan<N>>
 struct PlanInfo {
+    static constexpr int S2 = LastSplit<P>::S;
     static constexpr int B0 = N / P::R0;
     static constexpr int B1 = N / P::R1;
-    static constexpr int B2 = P::NST == 3 ? N / P::R2 : 0;
+    static constexpr int B2 = P::NST == 3 ? N / P::R2 / S2 : 0;   // work-items of the last stage
     static constexpr int TPL = B0 > B1 ? (B0 > B2 ? B0 : B2) : (B1 > B2 ? B1 : B2);
-    static constexpr int RMAX = P::R0 > P::R1 ? (P::R0 > P::R2 ? P::R0 : P::R2) : (P::R1 > P::R2 ? P::R1 : P::R2);
-    static constexpr int RLAST = P::NST == 3 ? P::R2 : P::R1;
+    static constexpr int RL2 = P::R2 * S2;
+    static constexpr int RMAX = P::R0 > P::R1 ? (P::R0 > RL2 ? P::R0 : RL2) : (P::R1 > RL2 ? P::R1 : RL2);
+    static constexpr int RLAST = P::NST == 3 ? RL2 : P::R1;
     static constexpr int BLAST = N / RLAST;
 };
 
@@ -278,10 +295,27 @@ __device__ __forceinline__ bool line_fft_stages(int j, const cpx<T>* __restrict_
         }
         __syncthreads();
         constexpr int NS2 = P::R0 * P::R1;
-        if (j < I::B2) {
+        if constexpr (I::S2 == 1) {
+            if (j < I::B2) {
 #pragma unroll
-            for (int r = 0; r < P::R2; ++r) v[r] = *lds(j + r * I::B2);
-            stage_bfly<N, P::R2, NS2, DIR, T>(v, j, tw);
+                for (int r = 0; r < P::R2; ++r) v[r] = *lds(j + r * I::B2);
+                stage_bfly<N, P::R2, NS2, DIR, T>(v, j, tw);
+            }
+        } else {
+            // butterflies j + h*B2 (h < S2); element r of butterfly jj is position jj + r*N/R2 =
+            // j + (h + S2*r)*B2, kept in v[h + S2*r]: output q at j + q*BLAST as for the others
+            if (j < I::B2) {
+#pragma unroll
+                for (int h = 0; h < I::S2; ++h) {
+                    const int jj = j + h * I::B2;
+                    cpx<T> u[P::R2];
+#pragma unroll
+                    for (int r = 0; r < P::R2; ++r) u[r] = *lds(jj + r * (N / P::R2));
+                    stage_bfly<N, P::R2, NS2, DIR, T>(u, jj, tw);
+#pragma unroll
+                    for (int r = 0; r < P::R2; ++r) v[h + I::S2 * r] = u[r];
+                }
+            }
         }
         return j < I::B2;
     } else {

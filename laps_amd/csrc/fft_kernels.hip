@@ -251,18 +251,20 @@ static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs
     return hipGetLastError();
 }
 
-// variant: bit 0 the plan (0 Plan, 1 PlanLow), bit 1 the tile width (inverse passes only)
+// variant: bit 0 the plan (0 Plan, 1 the low-radix plan: PlanLowW at 256, PlanLow otherwise),
+// bit 1 the tile width (inverse passes only)
 template <typename TI, typename T, int N, int DIR>
 static hipError_t do_strided(int variant, const cpx<TI>* in, cpx<T>* out, const StridedArgs& a, const cpx<T>* tw,
                              T scale, hipStream_t s) {
+    using Low = std::conditional_t<N == 256, PlanLowW<256>, PlanLow<N>>;
     const bool low = variant & 1;
     if constexpr (DIR > 0) {
         if (variant & 2) {
-            if (low) return do_strided_p<TI, T, N, DIR, PlanLow<N>, 1>(in, out, a, tw, scale, s);
+            if (low) return do_strided_p<TI, T, N, DIR, Low, 1>(in, out, a, tw, scale, s);
             return do_strided_p<TI, T, N, DIR, Plan<N>, 1>(in, out, a, tw, scale, s);
         }
     }
-    if (low) return do_strided_p<TI, T, N, DIR, PlanLow<N>, 0>(in, out, a, tw, scale, s);
+    if (low) return do_strided_p<TI, T, N, DIR, Low, 0>(in, out, a, tw, scale, s);
     return do_strided_p<TI, T, N, DIR, Plan<N>, 0>(in, out, a, tw, scale, s);
 }
 

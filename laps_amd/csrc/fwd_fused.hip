@@ -28,14 +28,21 @@ namespace laps {
 // the next at 4-5 waves per SIMD), except the FP32 y pass at 256, which runs faster on the
 // two-stage radix-16x16 plan (one LDS exchange, 16 values per work-item: 2.62 against 3.15
 // ms per mixed phase-shift step). On that plan the FP32 z pass ran at 2 waves per SIMD
-// (236 registers) and took 2.83 ms; on the low-radix plan 2.74 ms.
+// (236 registers) and took 2.83 ms; on the low-radix plan 2.74 ms, and 2.58 ms on its
+// variant with 32 busy work-items per line and 16 columns (PlanLowW; the FP64 passes on it
+// need 150-180 registers, one block per CU, or spill).
 template <typename T, int N, bool ZP>
 struct FwdCfg {
-    using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP), Plan<N>, PlanLow<N>>;
+    // the FP32 z pass at 256: the 8·8·4 plan with two last-stage butterflies per work-item
+    // (32 per line, none idle) and 16 columns, 128-byte rows of D, Fp and Uf
+    static constexpr bool WIDE = N == 256 && sizeof(T) == 4 && ZP;
+    using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP), Plan<N>,
+                                 std::conditional_t<WIDE, PlanLowW<N>, PlanLow<N>>>;
     using I = PlanInfo<N, P>;
     // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns), 256 bytes for the FP32
-    // y pass on the two-stage plan; the z pass keeps 8 FP32 columns (16 measured slower at 256)
-    static constexpr int TX = (sizeof(T) == 8) ? 8 : (ZP ? 8 : (N == 256 ? 32 : 16));
+    // y pass on the two-stage plan; the FP32 z pass has 16 columns only on the wide plan (on
+    // PlanLow, 1024-work-item blocks, it measured slower at 256), else 8
+    static constexpr int TX = (sizeof(T) == 8) ? 8 : (ZP ? (WIDE ? 16 : 8) : (N == 256 ? 32 : 16));
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
