@@ -310,6 +310,7 @@ def run_config2(steps):
             g.set_state(uu, pr)
             g.step(1)   # first step: plan autotune; not timed
             g.sync()
+            g.timing_enable(True)
             if prec == LAPS_FP64:
                 dts.append(g.scalars()["dt"])
                 t1 = time.perf_counter()
@@ -323,9 +324,15 @@ def run_config2(steps):
                     g.step(1)
             g.sync()
             sec = time.perf_counter() - t1
+            timing = g.timing()
+            g.timing_enable(False)
             sc = g.scalars()
-            out[key] = dict(steps_per_s=round((steps - 1) / sec, 3), time=sc["time"], radius=sc["radius"],
-                            max_divb=g.max_divb(), spectral=g.spectral())
+            ns = steps - 1
+            kern = {k: dict(ms_per_step=round(v[0] / ns, 4), gbs=round(v[1] / (v[0] * 1e-3) / 1e9, 1)
+                            if v[0] > 0 else 0.0, launches=v[2]) for k, v in timing.items() if v[2]}
+            out[key] = dict(steps_per_s=round(ns / sec, 3), time=sc["time"], radius=sc["radius"],
+                            max_divb=g.max_divb(), spectral=g.spectral(), kernels=kern,
+                            roofline=roofline(timing, "config2_" + key))
             log(f"config 2 {key}: {steps - 1} steps in {sec:.2f} s, t={sc['time']:.5f}, R={sc['radius']:.5f}")
     a, b = out["mixed"].pop("spectral"), out["fp64"].pop("spectral")
     rel = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))

@@ -297,8 +297,13 @@ __device__ __forceinline__ void r2c_pair2(const T (&fa0)[4], const T (&fb0)[4], 
     r2c_split<N, T>(v[1], line[1], lane, kx, oa1, ob1);
 }
 
+// waves per SIMD the register allocation must allow: at 512 the FP64 CFL variants need a few
+// more than 256 VGPRs otherwise, one wave per SIMD
+template <typename T, int N>
+constexpr int xf_min_waves() { return (sizeof(T) == 8 && N == 512) ? 2 : 1; }
+
 template <typename T, int N, bool FIRST, bool CFL, bool HALL, bool AEB>
-__global__ __launch_bounds__(LaneFft<N>::NT) void k_xfused(XFArgs a, const cpx<T>* __restrict__ twg, PhysParams pp,
+__global__ __launch_bounds__(LaneFft<N>::NT, (xf_min_waves<T, N>())) void k_xfused(XFArgs a, const cpx<T>* __restrict__ twg, PhysParams pp,
                                                 Ctl* __restrict__ ctl) {
     constexpr int LR = LaneFft<N>::LR;
     constexpr int NT = LaneFft<N>::NT;
