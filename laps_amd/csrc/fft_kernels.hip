@@ -30,7 +30,9 @@ struct StridedCfg {
     static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
-template <typename TI, typename T, int N, int DIR, class P, int W>
+// DUAL: both chains of a slab in one work-group (StridedArgs::dual), a separate instantiation so
+// that the second line's registers do not weigh on the other launches of the kernel
+template <typename TI, typename T, int N, int DIR, class P, int W, bool DUAL>
 __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const cpx<TI>* __restrict__ in,
                                                                      cpx<T>* __restrict__ out, StridedArgs a,
                                                                      const cpx<T>* __restrict__ tw, T scale,
@@ -42,7 +44,7 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
     in += blockIdx.y * a.in_bx;    // batched boxes: one per grid row
     out += blockIdx.y * a.out_bx;
     const cpx<double>* ph = a.ph;
-    if (a.nch == 2 && !a.dual) {
+    if (a.nch == 2 && !DUAL) {
         int ch;
         if (a.pair) {
             // blocks b and b + 8 run the same slab of chains 0 and 1: work-groups are dealt
@@ -104,7 +106,7 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
         if (valid && s >= 0 && !dead) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
     };
     cpx<T>* const stp = stw;
-    if (a.dual) {
+    if constexpr (DUAL) {
         // both chains: chain 0 from the input as loaded, chain 1 from the same values times the
         // grid shift, so the input is read and waited for once per slab
         using I = PlanInfo<N, P>;
@@ -246,8 +248,19 @@ static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs
     const long long nrest = (long long)ntile * a.nb1 * a.nb2;
     if (nrest <= 0) return hipSuccess;
     const long long nblk = (a.nch == 2 && !a.dual) ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest;
-    hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P, W>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0,
-                       s, in, out, a, tw, scale, ntile, nrest);
+    const dim3 grid((unsigned)nblk, a.nbox > 1 ? a.nbox : 1);
+    if constexpr (DIR > 0) {
+        if (a.dual) {
+            if (a.nch != 2 || !a.ph) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P, W, true>), grid, dim3(C::NT), 0, s, in, out, a, tw, scale,
+                               ntile, nrest);
+            return hipGetLastError();
+        }
+    } else {
+        if (a.dual) return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL((k_strided<TI, T, N, DIR, P, W, false>), grid, dim3(C::NT), 0, s, in, out, a, tw, scale, ntile,
+                       nrest);
     return hipGetLastError();
 }
 
