@@ -244,6 +244,14 @@ int laps_closure_eval(laps_ctx* ctx, float* outputs, double* rhs);
  * (xx, xy, xz, yy, yz, zz each); E = -u x B + (d_i/rho) J x B (3, J = curl B, d_i = 0 without
  * Hall); the energy flux (e + p + B^2/2) u - (u.B) B (3) (the fluxes of mhdrhs.f90:60-119). */
 int laps_closure_filter(laps_ctx* ctx, int r, double* out);
+/* The same training data under the sharp spectral cut-off of a coarser grid (no counterpart in
+ * the reference): the modes kx < ncx/2, |ky| < ncy/2, |kz| < ncz/2 of the fine context `ctx`,
+ * taken to the points of the coarse context `coarse` (same device, no finer grid) by its own
+ * inverse transform. out (ncx, ncy, ncz, 52): fields 0-25 the cut of the 26 fields above;
+ * fields 26-51 the cut of the same 26 fields formed, on the fine grid and so without aliasing,
+ * from the cut state (u = cut(rho u) / cut(rho), p from the cut e) and its current. Subgrid
+ * targets are the differences of fields 8-25 and 34-51. */
+int laps_closure_filter_spectral(laps_ctx* ctx, laps_ctx* coarse, double* out);
 int laps_mlp_forward(int device, int64_t n, const float* X, const float* params, float* Y);
 int laps_mlp_grad(int device, int64_t n, const float* X, const float* T, const float* params, float* grad,
                   float* loss);

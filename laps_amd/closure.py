@@ -45,6 +45,32 @@ def box_filter(f, r):
     return np.asfortranarray(g.mean(axis=(0, 2, 4)))
 
 
+def _cut_modes(n, nc):
+    """signed modes |m| < nc/2 (the coarse grid's Nyquist row dropped) as (fine, coarse) indices"""
+    h = nc // 2
+    m = np.concatenate([np.arange(0, h), np.arange(-(h - 1), 0)]) if h > 1 else np.zeros(1, dtype=np.int64)
+    return m % n, m % nc
+
+
+def spectral_cut(f, nc, fine_grid=False):
+    """Sharp spectral cut-off of a real (nx, ny, nz) field: the modes kx < ncx/2, |ky| < ncy/2,
+    |kz| < ncz/2, taken to the points of the (ncx, ncy, ncz) grid, or with fine_grid back to the
+    points of the fine grid (numpy restatement of laps_closure_filter_spectral's cut)."""
+    nx, ny, nz = f.shape
+    ncx, ncy, ncz = nc
+    F = np.fft.rfftn(f, axes=(1, 2, 0)) / (nx * ny * nz)
+    kx = np.arange(ncx // 2)
+    yf, yc = _cut_modes(ny, ncy)
+    zf, zc = _cut_modes(nz, ncz)
+    if fine_grid:
+        G = np.zeros_like(F)
+        G[np.ix_(kx, yf, zf)] = F[np.ix_(kx, yf, zf)]
+        return np.fft.irfftn(G, s=(ny, nz, nx), axes=(1, 2, 0)) * (nx * ny * nz)
+    G = np.zeros((ncx // 2 + 1, ncy, ncz), dtype=complex)
+    G[np.ix_(kx, yc, zc)] = F[np.ix_(kx, yf, zf)]
+    return np.fft.irfftn(G, s=(ncy, ncz, ncx), axes=(1, 2, 0)) * (ncx * ncy * ncz)
+
+
 def _k(n, L):
     return 2.0 * math.pi * np.fft.fftfreq(n, d=L / n)
 
@@ -120,13 +146,22 @@ def coarse_state(f, gamma):
     return uu_c, np.asfortranarray(np.concatenate([u_c, p_c[..., None]], axis=-1))
 
 
-def filtered_sample_device(g_fine, g_coarse, r):
+def filtered_sample_device(g_fine, g_coarse, r, spectral=False):
     """filtered_sample formed with the library on the device: the box filter of the fine state
     and of its flux products (laps_closure_filter on the fine context, with J = curl B from its
     own transforms), and the same products of the filtered state (the filter of width 1 on the
     coarse context, loaded with the filtered state). Targets are their differences: filter of
     the products minus products of the filter. Returns the coarse state, the raw features of
-    the coarse context and the (Nc, 18) targets; the coarse context keeps the filtered state."""
+    the coarse context and the (Nc, 18) targets; the coarse context keeps the filtered state.
+    With spectral, the filter is the sharp cut-off of the coarse grid instead
+    (laps_closure_filter_spectral), and the products of the cut state are formed on the fine
+    grid, free of aliasing."""
+    if spectral:
+        C, D = g_fine.closure_filter_spectral(g_coarse)
+        uu_c, prim_c = coarse_state(C, g_fine.params.adiabatic_index)
+        g_coarse.set_state(uu_c, prim_c)
+        T = (C[..., 8:26] - D[..., 8:26]).reshape(-1, 18, order="F")
+        return uu_c, prim_c, g_coarse.closure_features(), T
     f = g_fine.closure_filter(r)
     uu_c, prim_c = coarse_state(f, g_fine.params.adiabatic_index)
     g_coarse.set_state(uu_c, prim_c)
@@ -213,7 +248,7 @@ def check_finite(what, *arrays):
                                      f"(the fixed dt or the viscosity does not hold this run)")
 
 
-def fine_samples(p_fine, p_coarse, state, r, nsteps, every, dt):
+def fine_samples(p_fine, p_coarse, state, r, nsteps, every, dt, spectral=False):
     """Training samples of a fine run at a fixed dt, every `every` steps: (step, uu_c, prim_c,
     X, T) with the filtered state, its raw features and the subgrid targets, all formed on the
     device (filtered_sample_device); the fine state never leaves the GPU."""
@@ -223,12 +258,13 @@ def fine_samples(p_fine, p_coarse, state, r, nsteps, every, dt):
         for s in range(0, nsteps + 1, every):
             if s:
                 g.step(every)
-            yield (s,) + filtered_sample_device(g, gc, r)
+            yield (s,) + filtered_sample_device(g, gc, r, spectral)
 
 
 def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, every=32, amp=0.4,
-               di=0.05, visc=5e-3, train_steps=4000, ic="kolmogorov", dt_frac=0.8, log=print):
-    """Trains the closure on box-filtered fine runs and compares coarse runs with and
+               di=0.05, visc=5e-3, train_steps=4000, ic="kolmogorov", dt_frac=0.8, filt="box", log=print):
+    """Trains the closure on filtered fine runs (box filter, or with filt="spectral" the sharp
+    cut-off of the coarse grid) and compares coarse runs with and
     without it against the filtered fine run of another seed: energy spectra per shell up
     to the coarse grid's dealiased range, and the cost of a coarse step with the closure
     against the fine steps it stands in for."""
@@ -249,7 +285,7 @@ def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, 
     X, T = [], []
     for k, sd in enumerate(train_seeds):
         st = state0 if k == 0 else initial_state(pf, sd, amp, ic)
-        for s, uc, qc, x, t in fine_samples(pf, pc, st, r, nsteps, every, dt):
+        for s, uc, qc, x, t in fine_samples(pf, pc, st, r, nsteps, every, dt, filt == "spectral"):
             check_finite(f"seed {sd}, fine step {s}", uc, x, t)
             X.append(x)
             T.append(t.astype(np.float32))
@@ -269,7 +305,8 @@ def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, 
     # evaluation on an unseen seed: the filtered fine states at the start and after nsteps
     log(f"closure: evaluation run, seed {test_seed}, {nsteps} fine steps")
     snaps = []
-    for snap in fine_samples(pf, pc, initial_state(pf, test_seed, amp, ic), r, nsteps, max(1, nsteps // 4), dt):
+    for snap in fine_samples(pf, pc, initial_state(pf, test_seed, amp, ic), r, nsteps, max(1, nsteps // 4), dt,
+                             filt == "spectral"):
         check_finite(f"evaluation seed {test_seed}, fine step {snap[0]}", snap[1])
         snaps.append(snap[:3])
         log(f"closure: evaluation run at fine step {snap[0]} ({time.perf_counter() - t0:.1f} s)")
@@ -319,7 +356,8 @@ def experiment(n_fine=128, r=4, train_seeds=(11, 12), test_seed=21, nsteps=384, 
     cost = (tf * r) / res["closure"]["s_per_step"]
     icname = ("seeded k^-5/3 shells 1-8 rms" if ic != "alfvenic" else
               f"outward Alfven waves k^-1 shells 1-{n_fine // 16}, db/B0 =")
-    return dict(workload=f"fine {n_fine}^3 -> coarse {nc}^3 (box filter of {r}^3 cells), Hall d_i={di}, "
+    fname = f"box filter of {r}^3 cells" if filt != "spectral" else "sharp spectral cut-off of the coarse grid"
+    return dict(workload=f"fine {n_fine}^3 -> coarse {nc}^3 ({fname}), Hall d_i={di}, "
                          f"nu=eta={visc}, {icname} {amp}; trained on seeds {list(train_seeds)}, "
                          f"evaluated on seed {test_seed} after {nsteps} fine steps ({nsteps // r} coarse)",
                 samples=int(sum(1 for _ in range(0, nsteps + 1, every)) * len(train_seeds) * nc ** 3),
@@ -344,7 +382,9 @@ if __name__ == "__main__":
     ap.add_argument("--di", type=float, default=0.05)
     ap.add_argument("--ic", choices=["kolmogorov", "alfvenic"], default="kolmogorov")
     ap.add_argument("--dt-frac", type=float, default=0.8, help="fixed fine dt as a fraction of the first CFL dt")
+    ap.add_argument("--filter", choices=["box", "spectral"], default="box")
     a = ap.parse_args()
     print(json.dumps(experiment(n_fine=a.n_fine, r=a.ratio, nsteps=a.steps, every=a.every, visc=a.visc,
                                 train_steps=a.train_steps, amp=a.amp, di=a.di, ic=a.ic, dt_frac=a.dt_frac,
+                                filt=a.filter,
                                 log=lambda m: print(m, file=sys.stderr, flush=True))))

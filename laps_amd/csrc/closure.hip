@@ -335,6 +335,64 @@ __global__ void k_cl_filter(const double* __restrict__ S, const double* __restri
     for (int q = 0; q < kClFilt; ++q) out[q * Nc + ic] = acc[q] * inv;
 }
 
+// One field q of the 26 of k_cl_filter (same numbering and formulas) at every fine point, not
+// averaged: the input of the spectral (sharp cut-off) filter, one transform per field. Only the
+// inputs field q needs are read.
+__global__ void k_cl_point(const double* __restrict__ S, const double* __restrict__ J, long long N, int q, double di,
+                           double* __restrict__ out) {
+    const int pr[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        double v;
+        if (q < 8) {
+            v = S[q * N + i];
+        } else if (q < 14) {
+            v = S[(1 + pr[q - 8][0]) * N + i] * S[(8 + pr[q - 8][1]) * N + i];            // m_i u_j
+        } else if (q < 20) {
+            v = S[(4 + pr[q - 14][0]) * N + i] * S[(4 + pr[q - 14][1]) * N + i];          // B_i B_j
+        } else if (q < 23) {
+            const int a = q - 20, s1 = (a + 1) % 3, t1 = (a + 2) % 3;
+            const double us = S[(8 + s1) * N + i], ut = S[(8 + t1) * N + i];
+            const double bs = S[(4 + s1) * N + i], bt = S[(4 + t1) * N + i];
+            const double h = di / S[i];
+            v = -(us * bt - ut * bs) + h * (J[s1 * N + i] * bt - J[t1 * N + i] * bs);   // E = -u x B + (d_i/rho) J x B
+        } else {
+            const int a = q - 23;
+            double u[3], b[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                u[c] = S[(8 + c) * N + i];
+                b[c] = S[(4 + c) * N + i];
+            }
+            const double ub = u[0] * b[0] + u[1] * b[1] + u[2] * b[2];
+            const double et = S[7 * N + i] + S[11 * N + i] + 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+            v = et * u[a] - ub * b[a];                                                   // energy flux
+        }
+        out[i] = v;
+    }
+}
+
+// Sharp spectral cut-off between a fine and a coarse full spectrum (layout [z][y][KXP] with kx
+// fastest): the modes with kx < nxc/2, |my| < nyc/2 and |mz| < nzc/2 (signed; the coarse
+// Nyquist rows are dropped, so the cut is the same for +k and -k). dir > 0: gather into the
+// coarse spectrum C (zero outside the cut); dir < 0: scatter C into the fine spectrum F (the
+// caller zeroes F first).
+__global__ void k_spec_cut(cpx<double>* __restrict__ F, int nyf, int nzf, int kxpf, cpx<double>* __restrict__ C,
+                           int nxc, int nyc, int nzc, int kxpc, int dir) {
+    const long long Mc = (long long)kxpc * nyc * nzc;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < Mc; i += (long long)gridDim.x * blockDim.x) {
+        const int kx = (int)(i % kxpc);
+        const int ky = (int)((i / kxpc) % nyc);
+        const int kz = (int)(i / ((long long)kxpc * nyc));
+        const int my = ky <= nyc / 2 ? ky : ky - nyc;
+        const int mz = kz <= nzc / 2 ? kz : kz - nzc;
+        const bool kept = kx < nxc / 2 && 2 * my < nyc && -2 * my < nyc && 2 * mz < nzc && -2 * mz < nzc;
+        const long long f =
+            kept ? ((long long)(mz >= 0 ? mz : nzf + mz) * nyf + (my >= 0 ? my : nyf + my)) * kxpf + kx : 0;
+        if (dir > 0) C[i] = kept ? F[f] : cpx<double>{0.0, 0.0};
+        else if (kept) F[f] = C[i];
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 
 static unsigned blocks_for(long long n) { return (unsigned)((n + kThreadsCl - 1) / kThreadsCl); }
@@ -344,6 +402,22 @@ hipError_t launch_cl_features(const double* S, int nx, int ny, int nz, double dx
     const long long N = (long long)nx * ny * nz;
     hipLaunchKernelGGL(k_cl_features, dim3(blocks_for(N)), dim3(kThreadsCl), 0, s, S, nx, ny, nz, dx, dy, dz, norm,
                        X);
+    return hipGetLastError();
+}
+
+hipError_t launch_cl_point(const double* S, const double* J, long long N, int q, double di, double* out,
+                           hipStream_t s) {
+    const long long nb = blocks_for(N) < 65536 ? blocks_for(N) : 65536;
+    hipLaunchKernelGGL(k_cl_point, dim3((unsigned)nb), dim3(kThreadsCl), 0, s, S, J, N, q, di, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_spec_cut(cpx<double>* F, int nyf, int nzf, int kxpf, cpx<double>* C, int nxc, int nyc, int nzc,
+                           int kxpc, int dir, hipStream_t s) {
+    const long long Mc = (long long)kxpc * nyc * nzc;
+    const long long nb = blocks_for(Mc) < 65536 ? blocks_for(Mc) : 65536;
+    hipLaunchKernelGGL(k_spec_cut, dim3((unsigned)nb), dim3(kThreadsCl), 0, s, F, nyf, nzf, kxpf, C, nxc, nyc, nzc,
+                       kxpc, dir);
     return hipGetLastError();
 }
 

@@ -253,6 +253,12 @@ constexpr int kClIn = 28, kClH = 64, kClOut = 18;
 constexpr int kClNParam = kClIn * kClH + kClH + kClH * kClH + kClH + kClH * kClOut + kClOut;
 constexpr int kClNNorm = 2 * kClIn + kClOut;  // xmean, xstd, ystd
 constexpr int kClFilt = 26;  // box-filtered fields of laps_closure_filter
+// field q (0..25, the numbering of launch_cl_filter) at every point of S/J, unfiltered
+hipError_t launch_cl_point(const double* S, const double* J, long long N, int q, double di, double* out,
+                           hipStream_t s);
+// sharp spectral cut-off between full spectra: dir > 0 fine F -> coarse C, dir < 0 C -> F
+hipError_t launch_spec_cut(cpx<double>* F, int nyf, int nzf, int kxpf, cpx<double>* C, int nxc, int nyc, int nzc,
+                           int kxpc, int dir, hipStream_t s);
 hipError_t launch_cl_filter(const double* S, const double* J, int nx, int ny, int nz, int r, double di, double* out,
                             hipStream_t s);
 hipError_t launch_cl_features(const double* S, int nx, int ny, int nz, double dx, double dy, double dz,

@@ -1893,6 +1893,87 @@ int laps_closure_filter(laps_ctx* c, int r, double* out) {
     return done(LAPS_OK);
 }
 
+int laps_closure_filter_spectral(laps_ctx* c, laps_ctx* g, double* out) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (!g || !out) return fail(LAPS_EINVAL, "null coarse context or output");
+    if (g->p.device != c->p.device) return fail(LAPS_EINVAL, "the two contexts must be on one device");
+    if (g->nx > c->nx || g->ny > c->ny || g->nz > c->nz)
+        return fail(LAPS_EINVAL, "the coarse grid must not be finer than the fine one");
+    if ((rc = materialize(c))) return rc;
+    const bool fr = c->fresh;
+    cpx<double>* src = fr ? c->Ufull : c->U;
+    const Box& b = fr ? c->full : c->kept;
+    const long long fs = fr ? c->M : c->kb();
+    const long long N = c->N, Nc = g->N, Mc = g->full.modes();
+    const double di = c->p.if_hall ? c->p.ion_inertial_length : 0.0;
+    // device buffers: one fine field and its spectrum, J, the filtered state (with its
+    // primitives) and current on the fine grid, the 2 x 26 cut spectra, the coarse output
+    std::vector<void*> bufs;
+    auto get = [&](void** p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) return fail(LAPS_ENOMEM, "spectral filter buffer");
+        bufs.push_back(*p);
+        return LAPS_OK;
+    };
+    double *X, *J, *Sb, *Jb, *O;
+    cpx<double>*Fs, *Cs, *Tc;
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(g->stream);
+        for (void* p : bufs) (void)hipFree(p);
+        return code;
+    };
+    if ((rc = get((void**)&X, N * sizeof(double))) || (rc = get((void**)&J, 3 * N * sizeof(double))) ||
+        (rc = get((void**)&Sb, 12 * N * sizeof(double))) || (rc = get((void**)&Jb, 3 * N * sizeof(double))) ||
+        (rc = get((void**)&Fs, c->full.modes() * sizeof(cpx<double>))) ||
+        (rc = get((void**)&Cs, 2 * kClFilt * Mc * sizeof(cpx<double>))) ||
+        (rc = get((void**)&Tc, Mc * sizeof(cpx<double>))) || (rc = get((void**)&O, 2 * kClFilt * Nc * sizeof(double))))
+        return done(rc);
+    const int nyf = c->ny, nzf = c->nz, kxpf = c->full.KXP;
+    auto cut = [&](cpx<double>* C) {
+        return launch_spec_cut(Fs, nyf, nzf, kxpf, C, g->nx, g->ny, g->nz, g->full.KXP, +1, c->stream);
+    };
+    auto uncut = [&](const cpx<double>* C, double* real) -> int {
+        LAPS_CK(hipMemsetAsync(Fs, 0, c->full.modes() * sizeof(cpx<double>), c->stream));
+        LAPS_CK(launch_spec_cut(Fs, nyf, nzf, kxpf, (cpx<double>*)C, g->nx, g->ny, g->nz, g->full.KXP, -1, c->stream));
+        return inv_fp64(c, Fs, c->full, real);
+    };
+    hipError_t he = launch_curl(src, src + 8 * fs, fs, b, (fr ? c->tfull : c->tkept).view(), c->pp, c->ctl, c->stream);
+    if (he != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+    for (int a = 0; a < 3; ++a)
+        if ((rc = inv_fp64(c, src + (8 + a) * fs, b, J + a * N))) return done(rc);
+    // the cut of each field of the fine state: C[q]
+    for (int q = 0; q < kClFilt; ++q) {
+        if ((he = launch_cl_point(c->S, J, N, q, di, X, c->stream)) != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+        if ((rc = fwd_fp64(c, X, Fs))) return done(rc);
+        if ((he = cut(Cs + q * Mc)) != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+    }
+    // the cut state and current brought back to the fine grid (exact, zero-padded), the
+    // primitives of the cut state (u = cut(m) / cut(rho)), and the same fields from them: D[q]
+    for (int q = 0; q < 8; ++q)
+        if ((rc = uncut(Cs + q * Mc, Sb + q * N))) return done(rc);
+    LAPS_CK(launch_prim<double>(Sb, Sb + 8 * N, N, c->p.adiabatic_index, c->stream));
+    for (int a = 0; a < 3; ++a) {
+        if ((rc = fwd_fp64(c, J + a * N, Fs))) return done(rc);
+        if ((he = cut(Tc)) != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+        if ((rc = uncut(Tc, Jb + a * N))) return done(rc);
+    }
+    cpx<double>* Ds = Cs + kClFilt * Mc;
+    for (int q = 0; q < kClFilt; ++q) {
+        if ((he = launch_cl_point(Sb, Jb, N, q, di, X, c->stream)) != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+        if ((rc = fwd_fp64(c, X, Fs))) return done(rc);
+        if ((he = cut(Ds + q * Mc)) != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+    }
+    // to the coarse grid's points with the coarse context's transforms
+    LAPS_CK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 2 * kClFilt; ++q)
+        if ((rc = inv_fp64(g, Cs + q * Mc, g->full, O + q * Nc))) return done(rc);
+    he = hipStreamSynchronize(g->stream);
+    if (he == hipSuccess) he = hipMemcpy(out, O, 2 * kClFilt * Nc * sizeof(double), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return done(fail(LAPS_EDEVICE, hipGetErrorString(he)));
+    return done(LAPS_OK);
+}
+
 int laps_closure_eval(laps_ctx* c, float* out, double* rhs) {
     int rc = check_ctx(c, true);
     if (rc) return rc;

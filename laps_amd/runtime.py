@@ -25,7 +25,8 @@ EXPORTS = (
     "laps_get_spectral", "laps_moments", "laps_budgets", "laps_max_divb", "laps_fft_r2c", "laps_fft_c2r", "laps_sync",
     "laps_timing_enable", "laps_timing_read", "laps_get_stream",
     "laps_ens_create", "laps_ens_destroy", "laps_ens_size", "laps_ens_groups", "laps_ens_box", "laps_ens_step", "laps_ens_sync",
-    "laps_closure_set", "laps_closure_features", "laps_closure_eval", "laps_closure_filter", "laps_mlp_forward", "laps_mlp_grad",
+    "laps_closure_set", "laps_closure_features", "laps_closure_eval", "laps_closure_filter",
+    "laps_closure_filter_spectral", "laps_mlp_forward", "laps_mlp_grad",
     "laps_mlp_train", "laps_mlp_grad_noalloc",
 )
 
@@ -84,6 +85,7 @@ def lib():
         L.laps_closure_features.argtypes = [vp, _F]
         L.laps_closure_eval.argtypes = [vp, _F, _D]
         L.laps_closure_filter.argtypes = [vp, ctypes.c_int, _D]
+        L.laps_closure_filter_spectral.argtypes = [vp, vp, _D]
         L.laps_mlp_forward.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F]
         L.laps_mlp_grad.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, _F, _F]
         L.laps_mlp_train.argtypes = [ctypes.c_int, ctypes.c_int64, _F, _F, _F, ctypes.c_int, ctypes.c_int,
@@ -313,6 +315,14 @@ class Laps:
         out = np.zeros((self.nx // r, self.ny // r, self.nz // r, 26), order="F")
         _check(lib().laps_closure_filter(self._h, int(r), _fptr(out)))
         return out
+
+    def closure_filter_spectral(self, coarse):
+        """Sharp spectral cut-off of the state and its flux products onto the grid of the context
+        `coarse` (laps_closure_filter_spectral): two (ncx, ncy, ncz, 26) FP64 arrays, the cut of
+        the fine fields and the same fields formed from the cut state"""
+        out = np.zeros((coarse.nx, coarse.ny, coarse.nz, 52), order="F")
+        _check(lib().laps_closure_filter_spectral(self._h, coarse._h, _fptr(out)))
+        return out[..., :26], out[..., 26:]
 
     def fft_r2c(self, real):
         real = np.asfortranarray(real, dtype=np.float64)

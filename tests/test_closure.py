@@ -121,3 +121,26 @@ def test_network_sample_shapes_are_checked():
         runtime.mlp_grad(X, np.zeros((10, 15), np.float32), np.zeros(runtime.MLP_NPARAM, np.float32))
     with pytest.raises(runtime.LapsError, match="inputs"):
         runtime.mlp_forward(np.zeros((10, 27), np.float32), np.zeros(runtime.MLP_NPARAM, np.float32))
+
+
+def test_spectral_cut_closed_form():
+    """laps_amd.closure.spectral_cut on a sum of plane waves: the waves inside the cut
+    (kx < ncx/2, |ky| < ncy/2, |kz| < ncz/2) are sampled on the coarse grid unchanged, the rest
+    (here kx = 7 >= 4 and a coarse Nyquist row ky = 2) vanish; back on the fine grid the cut is
+    the same sum of the kept waves."""
+    from laps_amd import closure
+    nf, nc = (16, 8, 32), (8, 4, 8)
+
+    def grid(n):
+        return [np.arange(n[a]).reshape([-1 if b == a else 1 for b in range(3)]) * 2 * np.pi / n[a]
+                for a in range(3)]
+
+    def waves(x, y, z, kept_only):
+        f = 1.0 + np.cos(x) + np.sin(2 * x + y) + 0.5 * np.cos(3 * z - x) + 0 * x * y * z
+        if not kept_only:
+            f = f + np.cos(7 * x) + 0.25 * np.sin(2 * y + z)
+        return f
+
+    f = waves(*grid(nf), kept_only=False)
+    assert np.abs(closure.spectral_cut(f, nc) - waves(*grid(nc), kept_only=True)).max() < 1e-13
+    assert np.abs(closure.spectral_cut(f, nc, fine_grid=True) - waves(*grid(nf), kept_only=True)).max() < 1e-13

@@ -135,3 +135,46 @@ def test_device_filter_vs_numpy(rt):
         want = closure.box_filter(c, r)
         err = np.abs(f[..., q] - want).max() / max(np.abs(want).max(), 1e-30)
         assert err < 1e-11, (q, err)
+
+
+def test_device_spectral_filter_vs_numpy(rt):
+    """laps_closure_filter_spectral against numpy: the sharp cut of the 26 fields of a fine Hall
+    state onto a coarser grid, and of the same fields formed on the fine grid from the cut
+    state (u = cut(rho u)/cut(rho), p from cut(e), J = cut(curl B)); numpy transforms, the
+    definitions of laps_amd.closure (spectral_cut)."""
+    p = params((32, 16, 64), Ly=3.0, Lz=5.0, if_hall=1, ion_inertial_length=0.05)
+    pc = params((16, 16, 16), Ly=3.0, Lz=5.0, if_hall=1, ion_inertial_length=0.05)
+    uu, pr = seeded_state(p, seed=13, kmax=5, du=0.2, db=0.2)
+    L = (p.Lx, p.Ly, p.Lz)
+    nc = (pc.nx, pc.ny, pc.nz)
+    gam = p.adiabatic_index
+    with rt.Laps(p) as g, rt.Laps(pc) as gc:
+        g.set_state(uu, pr)
+        g.step(2)
+        gc.set_state(*seeded_state(pc, seed=1))
+        C, D = g.closure_filter_spectral(gc)
+        u2, q2 = g.state()
+
+    def fields(st, prim, J):
+        rho, m, B = st[..., 0], st[..., 1:4], st[..., 4:7]
+        u = prim[..., 0:3]
+        E = closure.emf(rho, u, B, J, 0.05)
+        F = closure.energy_flux(st[..., 7], u, B, prim[..., 3])
+        cols = [st[..., c] for c in range(8)]
+        cols += [m[..., i] * u[..., j] for i, j in closure.PAIRS]
+        cols += [B[..., i] * B[..., j] for i, j in closure.PAIRS]
+        return cols + [E[..., a] for a in range(3)] + [F[..., a] for a in range(3)]
+
+    J = closure.curl(u2[..., 4:7], L)
+    want_c = [closure.spectral_cut(c, nc) for c in fields(u2, q2, J)]
+    sb = np.stack([closure.spectral_cut(u2[..., v], nc, fine_grid=True) for v in range(8)], axis=-1)
+    mb = sb[..., 1:4]
+    ub = mb / sb[..., 0:1]
+    pb = (gam - 1.0) * (sb[..., 7] - 0.5 * (mb * ub).sum(-1) - 0.5 * (sb[..., 4:7] ** 2).sum(-1))
+    qb = np.concatenate([ub, pb[..., None]], axis=-1)
+    Jb = np.stack([closure.spectral_cut(J[..., a], nc, fine_grid=True) for a in range(3)], axis=-1)
+    want_d = [closure.spectral_cut(c, nc) for c in fields(sb, qb, Jb)]
+    for q in range(26):
+        for got, want, tag in ((C[..., q], want_c[q], "C"), (D[..., q], want_d[q], "D")):
+            err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-30)
+            assert err < 1e-11, (tag, q, err)
