@@ -52,14 +52,15 @@ struct FwdCfg {
     static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
-// derivative factors k' (real; the spectral factor is i*k'), mhdrhs.f90:192-204
-__device__ __forceinline__ void kprime_xy(double wx, double wy, const PhysParams& pp, double radius, double& kx,
+// derivative factors k' (real; the spectral factor is i*k'), mhdrhs.f90:192-204; r0r = R0/R,
+// formed once per work-item (the reference's (k*R0)/R per mode, within an ulp)
+__device__ __forceinline__ void kprime_xy(double wx, double wy, const PhysParams& pp, double r0r, double& kx,
                                           double& ky) {
     kx = wx;
-    ky = (wy * pp.radius0) / radius;
+    ky = wy * r0r;
     if (pp.corot_k) {
         kx = wx * pp.cos_c + wy * pp.sin_c;
-        ky = ((-wx * pp.sin_c + wy * pp.cos_c) * pp.radius0) / radius;
+        ky = (-wx * pp.sin_c + wy * pp.cos_c) * r0r;
     }
 }
 
@@ -108,15 +109,16 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT), (FwdCfg<T, N, false>::MI
     const bool valid = col < a.ncol;
     const int kx = valid ? col : 0;
     const double radius = ctl->radius;
+    const double r0r = pp.radius0 / radius;  // R0/R of the wave-number scalings
     const double wx = wxt[kx];
     // derivative factors kx', ky' of output position p of this column (mhdrhs.f90:192-204).
-    // FP64: the reference's expressions per mode (kprime_xy). FP32: kx' = cx + ax[p] and
+    // FP64: the reference's expressions per mode (kprime_xy, R0/R formed once). FP32: kx' = cx + ax[p] and
     // ky' = cy + ay[p], the y-dependent parts tabulated in LDS once per block (the pass
     // rounds the factors to FP32 either way), so no factor is held in registers across
     // transforms and no FP64 quotient is formed per mode.
     T cx = T(0), cy = T(0);
     if constexpr (TAB) {
-        const double rr = pp.radius0 / radius;
+        const double rr = r0r;
 #pragma unroll
         for (int i = 0; i < decltype(wyt)::PER; ++i) {
             const int t = (int)threadIdx.x + i * C::NT;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, false>::NT), (FwdCfg<T, N, false>::MI
             kyv = cy + say[p];
         } else {
             double kxp, kyp;
-            kprime_xy(wx, swy[p], pp, radius, kxp, kyp);
+            kprime_xy(wx, swy[p], pp, r0r, kxp, kyp);
             kxv = T(kxp);
             kyv = T(kyp);
         }
@@ -305,6 +307,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
     const bool valid = col < a.KX;
     const int kx = valid ? col : 0;
     const double radius = ctl->radius, tau = ctl->tau;
+    const double r0r = pp.radius0 / radius;  // R0/R of the wave-number scalings
     const double c = ctl->cc[irk], d = ctl->dd[irk], ts = ctl->ts[irk];
     const double wx = tb.wx[kx], wy = tb.wy[cy];
     const int lim = tb.kzlim[(long long)cy * a.KX + kx];
@@ -342,7 +345,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                             acc[r] = acc[r] + w[r];
                         } else {
                             const int p = j + r * I::BLAST;
-                            const T kz = T((swz[p] * pp.radius0) / radius);
+                            const T kz = T(swz[p] * r0r);
                             const cpx<T> q = mul_i(kz, w[r]);
                             acc[r] = m > 0 ? acc[r] + q : acc[r] - q;
                         }
@@ -428,7 +431,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                     const double fac = (v == 2 || v == 3) ? 3.0 : (v >= 5 ? 1.0 : 2.0);
                     fnl = fnl - cpx<double>{(u.x * fac) / tau, (u.y * fac) / tau};
                 }
-                const double k2 = (pp.visc || pp.resis) ? ksquare(wx, wy, wz, pp, ctl, radius) : 0.0;
+                const double k2 = (pp.visc || pp.resis) ? ksquare(wx, wy, wz, pp, ctl, r0r) : 0.0;
                 if (pp.visc && pp.visc_exp && v >= 1 && v <= 3)
                     fnl = fnl - cpx<double>{u.x * pp.nu * k2, u.y * pp.nu * k2};
                 if (pp.resis && pp.resis_exp && v >= 4 && v <= 6 && !(pp.conserve_bg && kx == 0 && p == 0))
@@ -443,11 +446,13 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                 Fp[v * a.ufs + mi[q]] = cpx<T>{T(fnl.x), T(fnl.y)};
                 if (pp.visc && !pp.visc_exp && v >= 1 && v <= 3) {
                     const double den = ts * k2 * pp.nu + 1.0;
-                    un = {un.x / den, un.y / den};
+                    const double inv = 1.0 / den;
+                    un = {un.x * inv, un.y * inv};
                 }
                 if (pp.resis && !pp.resis_exp && v >= 4 && v <= 6) {
                     const double den = ts * k2 * pp.eta + 1.0;
-                    un = {un.x / den, un.y / den};
+                    const double inv = 1.0 / den;
+                    un = {un.x * inv, un.y * inv};
                 }
                 if constexpr (OPT == 2) {
                     const double fx = tb.fx[kx], fy = tb.fy[cy], fz = tb.fz[cz];

@@ -60,16 +60,16 @@ __device__ __forceinline__ R cfl_point(R rho, R Bx, R By, R Bz, R p, R ux, R uy,
 
 // k_square: grid_initialize form until the first evolve_radius, then
 // update_ksquare (mhdinit.f90:114-122; AEBmod.f90:87-125); wave numbers w*
+// r0r = R0/R, formed once per work-item (the reference's (k*R0)/R per mode, within an ulp)
 __device__ __forceinline__ double ksquare(double kx, double ky, double kz, const PhysParams& pp, const Ctl* ctl,
-                                          double radius) {
+                                          double r0r) {
     if (!ctl->ksq_updated) return kx * kx + ky * ky + kz * kz;
-    const double R0 = pp.radius0, R = radius;
     if (pp.corot_ksq) {
-        const double c = pp.cos_c, s = pp.sin_c, r0r = R0 / R;
-        const double a = s * R0 / R, b = c * R0 / R, z = kz * R0 / R;
+        const double c = pp.cos_c, s = pp.sin_c;
+        const double a = s * r0r, b = c * r0r, z = kz * r0r;
         return kx * kx * (c * c + a * a) + ky * ky * (s * s + b * b) + kx * ky * 2 * c * s * (1 - r0r * r0r) + z * z;
     }
-    const double a = ky * R0 / R, b = kz * R0 / R;
+    const double a = ky * r0r, b = kz * r0r;
     return kx * kx + a * a + b * b;
 }
 

@@ -330,7 +330,7 @@ __global__ __launch_bounds__(LaneFft<N>::NT, (xf_min_waves<T, N>())) void k_xfus
     long long row = blk * RPB + (rib < RPB ? rib : 0);
     const bool valid = rib < RPB && row < a.nrows;
     if (row >= a.nrows) row = a.nrows - 1;  // computes a duplicate row, never stored
-    cpx<T>* const line1[1] = {sh + 2 * rib * LS};
+    [[maybe_unused]] cpx<T>* const line1[1] = {sh + 2 * rib * LS};
     cpx<T>* const line2[2] = {sh + 2 * rib * LS, sh + (2 * rib + 1) * LS};
     // the twiddle table is fetched first and written to LDS once the row's first
     // loads are in flight, so the wait for it does not wait for them
