@@ -31,24 +31,27 @@ namespace laps {
 // (236 registers) and took 2.83 ms; on the low-radix plan 2.74 ms, and 2.58 ms on its
 // variant with 32 busy work-items per line and 16 columns (PlanLowW; the FP64 passes on it
 // need 150-180 registers, one block per CU, or spill).
-template <typename T, int N, bool ZP>
+// ALT (the FP32 y pass at 256 only): the wide 8·8·4 plan with 16 columns instead of the
+// two-stage plan with 32, for launches of one chain (0.82 -> 0.79 ms per mixed 2/3-rule step;
+// with the two chains of phase shift the two-stage plan is faster, 2.47 against 2.69)
+template <typename T, int N, bool ZP, bool ALT = false>
 struct FwdCfg {
     // the FP32 z pass at 256: the 8·8·4 plan with two last-stage butterflies per work-item
     // (32 per line, none idle) and 16 columns, 128-byte rows of D, Fp and Uf
-    static constexpr bool WIDE = N == 256 && sizeof(T) == 4 && ZP;
-    using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP), Plan<N>,
+    static constexpr bool WIDE = N == 256 && sizeof(T) == 4 && (ZP || ALT);
+    using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP && !ALT), Plan<N>,
                                  std::conditional_t<WIDE, PlanLowW<N>, PlanLow<N>>>;
     using I = PlanInfo<N, P>;
     // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns), 256 bytes for the FP32
     // y pass on the two-stage plan; the FP32 z pass has 16 columns only on the wide plan (on
     // PlanLow, 1024-work-item blocks, it measured slower at 256), else 8
-    static constexpr int TX = (sizeof(T) == 8) ? 8 : (ZP ? (WIDE ? 16 : 8) : (N == 256 ? 32 : 16));
+    static constexpr int TX = (sizeof(T) == 8) ? 8 : (WIDE ? 16 : (ZP ? 8 : (N == 256 ? 32 : 16)));
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
     // waves per SIMD the register allocation must allow: the FP32 y pass at 256 (512-thread
     // blocks, 66 KB of LDS) fits two blocks per CU only at <= 128 VGPRs
-    static constexpr int MINW = (sizeof(T) == 4 && N == 256 && !ZP) ? 4 : 1;
+    static constexpr int MINW = (sizeof(T) == 4 && N == 256 && !ZP && !ALT) ? 4 : 1;
     static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
@@ -68,14 +71,14 @@ template <typename T>
 __device__ __forceinline__ cpx<T> mul_i(T k, cpx<T> z) { return {-k * z.y, k * z.x}; }  // (i k) z
 
 // ---------------------------------------------------------------- y pass
-template <typename T, int N>
-__global__ __launch_bounds__((FwdCfg<T, N, false>::NT), (FwdCfg<T, N, false>::MINW)) void k_yfwd_group(const cpx<T>* __restrict__ in,
+template <typename T, int N, bool ALT>
+__global__ __launch_bounds__((FwdCfg<T, N, false, ALT>::NT), (FwdCfg<T, N, false, ALT>::MINW)) void k_yfwd_group(const cpx<T>* __restrict__ in,
                                                                  cpx<T>* __restrict__ out, YGArgs a,
                                                                  const cpx<T>* __restrict__ tw,
                                                                  const double* __restrict__ wxt,
                                                                  const double* __restrict__ wyf, PhysParams pp,
                                                                  const Ctl* __restrict__ ctl, int ntile, int nz) {
-    using C = FwdCfg<T, N, false>;
+    using C = FwdCfg<T, N, false, ALT>;
     using I = typename C::I;
     // FP32: the derivative factors come from LDS tables of their y-dependent parts (below)
     constexpr bool TAB = sizeof(T) == 4;
@@ -467,15 +470,24 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
 
 // ---------------------------------------------------------------- launchers
 
+template <typename T, int N, bool ALT>
+static hipError_t do_yg_c(const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw, const double* wxt,
+                          const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz, hipStream_t s) {
+    using C = FwdCfg<T, N, false, ALT>;
+    const int ntile = (a.ncol + C::TX - 1) / C::TX;
+    const long long nblk = (long long)ntile * nz * a.ngroups * (a.nch == 2 ? 2 : 1);
+    hipLaunchKernelGGL((k_yfwd_group<T, N, ALT>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0, s, in,
+                       out, a, tw, wxt, wyf, pp, ctl, ntile, nz);
+    return hipGetLastError();
+}
+
 template <typename T, int N>
 static hipError_t do_yg(const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw, const double* wxt,
                         const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz, hipStream_t s) {
-    using C = FwdCfg<T, N, false>;
-    const int ntile = (a.ncol + C::TX - 1) / C::TX;
-    const long long nblk = (long long)ntile * nz * a.ngroups * (a.nch == 2 ? 2 : 1);
-    hipLaunchKernelGGL((k_yfwd_group<T, N>), dim3((unsigned)nblk, a.nbox > 1 ? a.nbox : 1), dim3(C::NT), 0, s, in, out,
-                       a, tw, wxt, wyf, pp, ctl, ntile, nz);
-    return hipGetLastError();
+    if constexpr (sizeof(T) == 4 && N == 256) {
+        if (a.nch != 2) return do_yg_c<T, N, true>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+    }
+    return do_yg_c<T, N, false>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
 }
 
 template <typename T>
