@@ -88,11 +88,52 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
         // is skipped whole, before any barrier (about a quarter of the tiles of a mask)
         if (__ballot(valid && !dead) == 0) return;
     }
+    // Hall current from B (curl): component jc of J = i k' x B, from fields 4 + (jc+1)%3 and
+    // 4 + (jc+2)%3 of this slab's column (the evaluation of k_curl, mhdrhs.f90:313-339)
+    const int jc = (a.curl && b2 >= 8) ? b2 - 8 : -1;
+    const cpx<TI>* ibs = ib;
+    const cpx<TI>* ibt = ib;
+    double kxy[2] = {0.0, 0.0};  // the k' components of index (jc+1)%3 and (jc+2)%3 that do not depend on kz
+    double rz = 1.0;             // R0/R
+    int sz = 0;                  // which of the two is kz' (0: none)
+    if (jc >= 0) {
+        const int c1 = (jc + 1) % 3, c2 = (jc + 2) % 3;
+        ibs = ib + (long long)(4 + c1 - b2) * a.in_b2;
+        ibt = ib + (long long)(4 + c2 - b2) * a.in_b2;
+        rz = a.k_r0 / (a.kctl + blockIdx.y)->radius;
+        const double wx = a.kwx[valid ? col : 0], wy = a.kwy[b1];
+        double kx = wx, ky = wy * rz;
+        if (a.k_corot) {
+            kx = wx * a.k_cos + wy * a.k_sin;
+            ky = (-wx * a.k_sin + wy * a.k_cos) * rz;
+        }
+        const double kk[3] = {kx, ky, 0.0};
+        kxy[0] = kk[c1];
+        kxy[1] = kk[c2];
+        sz = c1 == 2 ? 1 : (c2 == 2 ? 2 : 0);
+    }
+    // J_jc = i (k_c1 B_c2 - k_c2 B_c1), kz' = wz R0/R of stored z row s (the reference's
+    // (w R0)/R within an ulp)
+    auto curl_at = [&](int s) -> cpx<T> {
+        const double kz = a.kwz[s] * rz;
+        const double k1 = sz == 1 ? kz : kxy[0], k2 = sz == 2 ? kz : kxy[1];
+        const cpx<TI> b1v = ibs[s * in_ls], b2v = ibt[s * in_ls];
+        const cpx<double> u{double(b2v.x) * k1 - double(b1v.x) * k2, double(b2v.y) * k1 - double(b1v.y) * k2};
+        return cpx<T>{T(-u.y), T(u.x)};
+    };
     auto load = [&](int p) -> cpx<T> {
         const int s = pos_to_stored(p, N, in_cnt);
         if (!valid || s < 0 || dead) return cpx<T>{T(0), T(0)};
         if (a.row_role == 2 && mask_row_dead(a.mask, a.mX, a.mY, a.mZ, col, p <= N / 2 ? p : p - N))
             return cpx<T>{T(0), T(0)};
+        if (jc >= 0) {
+            const cpx<T> v = curl_at(s);
+            if (ph) {
+                const cpx<double> w = cmul(ph[p], wcb);
+                return cmul(v, cpx<T>{T(w.x), T(w.y)});
+            }
+            return v;
+        }
         const cpx<TI> v = ib[s * in_ls];
         if (ph) {
             const cpx<double> w = cmul(ph[p], wcb);
@@ -118,8 +159,12 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
                 const int s = pos_to_stored(p, N, in_cnt);
                 cpx<T> x{T(0), T(0)};
                 if (valid && s >= 0 && !dead) {
-                    const cpx<TI> q = ib[s * in_ls];
-                    x = cpx<T>{T(q.x), T(q.y)};
+                    if (jc >= 0) {
+                        x = curl_at(s);
+                    } else {
+                        const cpx<TI> q = ib[s * in_ls];
+                        x = cpx<T>{T(q.x), T(q.y)};
+                    }
                 }
                 v[r] = x;
                 const cpx<double> sh1 = cmul(ph[p], wcb);

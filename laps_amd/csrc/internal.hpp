@@ -13,6 +13,8 @@ namespace laps {
 // Strided batched C2C along one axis. Lines run along the axis with element
 // stride `ls`; `ncol` adjacent lines (stride 1) form a column tile; two batch
 // dimensions select the tile. Stored positions may be pruned (cnt < N).
+struct Ctl;
+
 struct StridedArgs {
     int ncol;
     int in_cnt, out_cnt;
@@ -47,6 +49,16 @@ struct StridedArgs {
     // stride): the launch covers nbox boxes (grid y), box b at in + b * in_bx, out + b * out_bx
     int nbox;
     long long in_bx, out_bx;
+    // curl (z inverse of a kept-box state with Hall): input fields 8-10 are not read; they are
+    // J = i k' x B formed from fields 4-6 as they are loaded (k' of the kept-box tables kw*,
+    // with the expanding-box / corotating scaling of the box's control block, one per grid row)
+    int curl;
+    const double* kwx;
+    const double* kwy;
+    const double* kwz;
+    const Ctl* kctl;
+    double k_r0, k_cos, k_sin;
+    int k_corot;
 };
 
 // true when no mode of row (mx, my) is kept: its mz = 0 mode is outside the mask (the

@@ -403,7 +403,7 @@ template <typename T, typename TS = double>
 int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, int nf, T* dst, cpx<T>* A,
                  cpx<T>* B, int plan_z = 0, int plan_y = 0, bool shift = false, int nch = 1, long long a_ch = 0,
                  long long b_ch = 0, bool masked = false, int nbox = 1, long long src_bx = 0, long long a_bx = 0,
-                 long long b_bx = 0) {
+                 long long b_bx = 0, bool curl = false) {
     if (nch == 2 && (dst || !shift)) return fail(LAPS_EINVAL, "internal: two chains are the shifted pair");
     const long long NX = c->nx, NY = c->ny, NZ = c->nz;
     StridedArgs z{};
@@ -420,6 +420,14 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
     z.nch = nch; z.out_ch = a_ch; z.pair = true;
     z.dual = nch == 2;  // the shifted chain from the plain chain's loads, in the same work-group
     z.nbox = nbox; z.in_bx = src_bx; z.out_bx = a_bx;
+    if (curl) {  // the Hall current formed from B as the z pass loads it (no k_curl launch)
+        const BoxTables t = c->tkept.view();
+        z.curl = 1;
+        z.kwx = t.wx; z.kwy = t.wy; z.kwz = t.wz;
+        z.kctl = c->ctl;
+        z.k_r0 = c->pp.radius0; z.k_cos = c->pp.cos_c; z.k_sin = c->pp.sin_c;
+        z.k_corot = c->pp.corot_k;
+    }
     // masked: src is the kept-box state after a masked substage (zero outside the kept set);
     // the mask rows with no kept mode are skipped by the z inverse and read as zero by the y
     // inverse
@@ -429,8 +437,9 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
         z.mask = mask; z.row_role = 1; z.cy_cnt = b.CY;
         z.mX = NX; z.mY = NY; z.mZ = NZ;
     }
-    const double ez = (double)nf * rows;
-    LAPS_CK(timed(c, K_ZINV, ez * (b.CZ * 2.0 * sizeof(TS) + nch * NZ * 2.0 * sizeof(T)),
+    // bytes: every field's kept rows read (J's three as two B fields each) and written per chain
+    const double ez = (double)nf * rows, er = (double)(nf + (curl ? 3 : 0)) * rows;
+    LAPS_CK(timed(c, K_ZINV, er * b.CZ * 2.0 * sizeof(TS) + ez * nch * NZ * 2.0 * sizeof(T),
                   [&] { return launch_strided<TS, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
     y.ncol = b.KX; y.in_cnt = b.CY; y.out_cnt = (int)NY;
@@ -544,17 +553,11 @@ int substage(laps_ctx* c, int irk) {
         }
         xin = 96.0 * c->N + (hall ? 3.0 * xa.nrows * c->nxh * 2 * sizeof(T) : 0.0);
     } else {
-        if (hall)
-            LAPS_CK(timed(c, K_CURL, (48.0 + 6.0 * sizeof(T)) * KB, [&] {
-                if (c->Uf)
-                    return launch_curl(c->U, c->Uf + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream,
-                                       nb, bt.u, bt.uf);
-                return launch_curl(c->U, c->U + 8 * KB, KB, c->kept, c->tkept.view(), c->pp, c->ctl, c->stream, nb,
-                                   bt.u, bt.u);
-            }));
+        // the Hall current J = i k' x B is formed by the z inverse from the B fields it loads
+        // (StridedArgs::curl): no J is written to or read from HBM
         if ((rc = inv_pipeline<T, T>(c, state_src<T>(c), KB, k, c->nf_in(), (T*)nullptr, P1, P2, c->plan_zinv,
                                      c->plan_yinv, nch == 2, nch, c->p1, c->p2, true, nb,
-                                     c->Uf ? bt.uf : bt.u, bt.p1, bt.p2)))
+                                     c->Uf ? bt.uf : bt.u, bt.p1, bt.p2, hall)))
             return rc;
         xa.nch = nch;
         xa.st_ch = c->p2;
@@ -705,7 +708,7 @@ int autotune(laps_ctx* c) {
             float ms_zy = 0, ms_y = 0;
             int rc = time_it([&] {
                 return inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, v, 0,
-                                          nch == 2, nch, c->p1, c->p2, true);
+                                          nch == 2, nch, c->p1, c->p2, true, 1, 0, 0, 0, c->p.if_hall != 0);
             }, ms_zy);
             if (!rc) rc = time_it([&] {
                 const hipError_t he = launch_strided<T, T>(c->ny, +1, P1, P2, y, tw_of<T>(c, 1), T(1), c->stream, v);
@@ -738,7 +741,7 @@ int autotune(laps_ctx* c) {
             float ms = 0;
             const int rc = time_it([&] {
                 return inv_pipeline<T, T>(c, state_src<T>(c), c->kb(), k, c->nf_in(), (T*)nullptr, P1, P2, vz, v,
-                                          nch == 2, nch, c->p1, c->p2, true);
+                                          nch == 2, nch, c->p1, c->p2, true, 1, 0, 0, 0, c->p.if_hall != 0);
             }, ms);
             if (rc) {
                 (void)hipEventDestroy(e0);
