@@ -446,7 +446,9 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                     const cpx<double> fp{double(fq[q].x), double(fq[q].y)};
                     un = cpx<double>{c * fnl.x + d * fp.x, c * fnl.y + d * fp.y} + u;
                 }
-                Fp[v * a.ufs + mi[q]] = cpx<T>{T(fnl.x), T(fnl.y)};
+                // the RHS of the last substage is never read: the next step's first substage
+                // weighs the previous RHS by dd(0) = 0 and reads none (rktmod.f90:40-42)
+                if (irk != 2) Fp[v * a.ufs + mi[q]] = cpx<T>{T(fnl.x), T(fnl.y)};
                 if (pp.visc && !pp.visc_exp && v >= 1 && v <= 3) {
                     const double den = ts * k2 * pp.nu + 1.0;
                     const double inv = 1.0 / den;

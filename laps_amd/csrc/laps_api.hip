@@ -635,13 +635,13 @@ int substage(laps_ctx* c, int irk) {
         za.phb = c->phyk;
     }
     const T scale = T(1.0 / ((double)c->nx * (double)c->ny * (double)c->nz));
-    // U read + write (FP64), Fp write (+ read when irk > 0) in the transform type
+    // U read + write (FP64), Fp in the transform type: read when irk > 0, written when irk < 2
     // (kept modes only: outside the kept set neither U nor Fp is read, and U is written as
     // zero only by the first substage after a state upload)
     za.clear_outside = first ? 1 : 0;
     const double uw = first ? (double)k.KX * k.CY * k.CZ : (double)c->nkept;
     const double zb = c->zfwd_bytes * sizeof(T) * (za.Ds ? 2 : 1) +
-                      8.0 * (16.0 * uw + (double)c->nkept * (16.0 + 2.0 * sizeof(T) * (irk ? 2 : 1))) +
+                      8.0 * (16.0 * uw + (double)c->nkept * (16.0 + 2.0 * sizeof(T) * (irk == 1 ? 2 : 1))) +
                       (c->Uf ? 8.0 * uw * 8.0 : 0.0);
     LAPS_CK(timed(c, K_ZFWD, zb, [&] {
         return launch_zfwd_rhs<T>(c->nz, P2, c->U, (cpx<T>*)c->Fp, za, tw_of<T>(c, 2), c->tkept.view(), c->tfull.wz, c->pp,
