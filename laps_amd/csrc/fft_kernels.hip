@@ -61,6 +61,17 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
         out += ch * a.out_ch;
         if (ch == 0) ph = nullptr;
     }
+    if (a.curl) {
+        // J formed here: the nb2 field blocks of a slab (tile, b1) run close together on one
+        // XCD, so the J blocks' reads of the B fields hit that XCD's L2. Work-groups are dealt
+        // round-robin over the 8 XCDs; slab s, field f is block ((s / 8) nb2 + f) 8 + s % 8
+        // (the grid is padded to whole groups of 8 slabs).
+        const long long ns = (long long)ntile * a.nb1;
+        const long long g = bid >> 3;
+        const long long sl = (g / a.nb2) * 8 + (bid & 7);
+        if (sl >= ns) return;  // whole work-group, before any barrier
+        bid = (g % a.nb2) * ns + sl;
+    }
     LdsTable<cpx<T>, N, C::NT> twt;
     twt.fetch(tw);
     const int t = threadIdx.x;
@@ -292,7 +303,10 @@ static hipError_t do_strided_p(const cpx<TI>* in, cpx<T>* out, const StridedArgs
     const int ntile = (a.ncol + C::TX - 1) / C::TX;
     const long long nrest = (long long)ntile * a.nb1 * a.nb2;
     if (nrest <= 0) return hipSuccess;
-    const long long nblk = (a.nch == 2 && !a.dual) ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest;
+    if (a.curl && (DIR < 0 || (a.nch == 2 && !a.dual))) return hipErrorInvalidValue;
+    const long long nslab = (long long)ntile * a.nb1;
+    const long long nblk = a.curl ? (nslab + 7) / 8 * 8 * a.nb2
+                                  : ((a.nch == 2 && !a.dual) ? 2 * (a.pair ? (nrest + 7) / 8 * 8 : nrest) : nrest);
     const dim3 grid((unsigned)nblk, a.nbox > 1 ? a.nbox : 1);
     if constexpr (DIR > 0) {
         if (a.dual) {

@@ -437,8 +437,9 @@ int inv_pipeline(laps_ctx* c, const cpx<TS>* src, long long fs, const Box& b, in
         z.mask = mask; z.row_role = 1; z.cy_cnt = b.CY;
         z.mX = NX; z.mY = NY; z.mZ = NZ;
     }
-    // bytes: every field's kept rows read (J's three as two B fields each) and written per chain
-    const double ez = (double)nf * rows, er = (double)(nf + (curl ? 3 : 0)) * rows;
+    // bytes: the kept rows of every stored field read once, every field written per chain (with
+    // the Hall current formed here, J is not read: its B reads are those of fields 4-6)
+    const double ez = (double)nf * rows, er = (double)(nf - (curl ? 3 : 0)) * rows;
     LAPS_CK(timed(c, K_ZINV, er * b.CZ * 2.0 * sizeof(TS) + ez * nch * NZ * 2.0 * sizeof(T),
                   [&] { return launch_strided<TS, T>(c->nz, +1, src, A, z, tw_of<T>(c, 2), T(1), c->stream, plan_z); }));
     StridedArgs y{};
