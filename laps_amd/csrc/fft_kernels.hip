@@ -10,6 +10,8 @@
 //               adjacent work-items touch adjacent columns, so each HBM
 //               access of a row of the tile is one 128-byte segment.
 //               Input and output rows may be stored pruned (kept ky/kz only).
+//               The z inverse of a Hall state forms the current J = ik' x B
+//               from the B fields as it loads them (no J is stored).
 //  k_x_c2r    : x C2R of two rows at once (Z = A + iB, one complex line).
 //  k_x_r2c    : x R2C of two rows at once, split with the conjugate trick.
 #include "internal.hpp"

@@ -230,14 +230,11 @@ hipError_t launch_zfwd_rhs(int nz, const cpx<T>* D, cpx<double>* U, cpx<T>* Fp, 
                            const Ctl* ctl, int irk, T scale, hipStream_t s);
 
 // --- physics launchers (physics_kernels.hip) ---
-// The step's small kernels take nbox batched boxes (ensemble groups): box b's state at
-// B + b * b_bx, current at J + b * j_bx, control block ctl + b.
+// The step's control kernels take nbox batched boxes (ensemble groups): control block ctl + b.
+// J = i k' x B of one box (the first substage after a state upload, the closure's filters;
+// later substages form J in the z inverse)
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fstride, Box b, BoxTables t,
-                       const PhysParams& pp, const Ctl* ctl, hipStream_t s, int nbox = 1, long long b_bx = 0,
-                       long long j_bx = 0);
-hipError_t launch_curl(const cpx<double>* B, cpx<float>* J, long long fstride, Box b, BoxTables t,
-                       const PhysParams& pp, const Ctl* ctl, hipStream_t s, int nbox = 1, long long b_bx = 0,
-                       long long j_bx = 0);
+                       const PhysParams& pp, const Ctl* ctl, hipStream_t s);
 hipError_t launch_ctl_begin(Ctl* ctl, hipStream_t s, int nbox = 1);
 hipError_t launch_ctl_finalize(Ctl* ctl, const PhysParams& pp, hipStream_t s, int nbox = 1);
 hipError_t launch_ctl_end(Ctl* ctl, const PhysParams& pp, hipStream_t s, int nbox = 1);

@@ -50,14 +50,9 @@ __device__ __forceinline__ bool box_index(long long idx, const Box& b, int& kx, 
 }
 
 // ------------------------------------------------------------------ curl
-// J is written in FP64, or rounded once to FP32 for the mixed-precision step's z-inverse input
-template <typename TJ>
-__global__ void k_curl(const cpx<double>* __restrict__ U, cpx<TJ>* __restrict__ J, long long fs, Box b,
-                       BoxTables t, PhysParams pp, const Ctl* __restrict__ ctl, long long b_bx, long long j_bx) {
-    // batched boxes: one per grid row
-    U += blockIdx.y * b_bx;
-    J += blockIdx.y * j_bx;
-    ctl += blockIdx.y;
+// J = i k' x B on the modes of box b (FP64)
+__global__ void k_curl(const cpx<double>* __restrict__ U, cpx<double>* __restrict__ J, long long fs, Box b,
+                       BoxTables t, PhysParams pp, const Ctl* __restrict__ ctl) {
     const long long n = b.modes();
     const double radius = ctl->radius;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -65,25 +60,15 @@ __global__ void k_curl(const cpx<double>* __restrict__ U, cpx<TJ>* __restrict__ 
         if (!box_index(i, b, kx, cy, cz)) continue;
         const KVec k = kvec(t.wx[kx], t.wy[cy], t.wz[cz], pp, radius);
         const cpx<double> bx = U[4 * fs + i], by = U[5 * fs + i], bz = U[6 * fs + i];
-        const cpx<double> jx = ik(k.ky, bz) - ik(k.kz, by), jy = ik(k.kz, bx) - ik(k.kx, bz);
-        const cpx<double> jz = ik(k.kx, by) - ik(k.ky, bx);
-        J[i] = cpx<TJ>{TJ(jx.x), TJ(jx.y)};
-        J[fs + i] = cpx<TJ>{TJ(jy.x), TJ(jy.y)};
-        J[2 * fs + i] = cpx<TJ>{TJ(jz.x), TJ(jz.y)};
+        J[i] = ik(k.ky, bz) - ik(k.kz, by);
+        J[fs + i] = ik(k.kz, bx) - ik(k.kx, bz);
+        J[2 * fs + i] = ik(k.kx, by) - ik(k.ky, bx);
     }
 }
 
 hipError_t launch_curl(const cpx<double>* B, cpx<double>* J, long long fs, Box b, BoxTables t, const PhysParams& pp,
-                       const Ctl* ctl, hipStream_t s, int nbox, long long b_bx, long long j_bx) {
-    hipLaunchKernelGGL(k_curl<double>, dim3(grid_for(b.modes()), nbox > 1 ? nbox : 1), dim3(kThreads), 0, s, B, J, fs, b,
-                       t, pp, ctl, b_bx, j_bx);
-    return hipGetLastError();
-}
-
-hipError_t launch_curl(const cpx<double>* B, cpx<float>* J, long long fs, Box b, BoxTables t, const PhysParams& pp,
-                       const Ctl* ctl, hipStream_t s, int nbox, long long b_bx, long long j_bx) {
-    hipLaunchKernelGGL(k_curl<float>, dim3(grid_for(b.modes()), nbox > 1 ? nbox : 1), dim3(kThreads), 0, s, B, J, fs, b,
-                       t, pp, ctl, b_bx, j_bx);
+                       const Ctl* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_curl, dim3(grid_for(b.modes())), dim3(kThreads), 0, s, B, J, fs, b, t, pp, ctl);
     return hipGetLastError();
 }
 
