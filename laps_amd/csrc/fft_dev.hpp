@@ -34,6 +34,15 @@ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
 template <typename T>
 __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
 
+// one complex value stored with the non-temporal hint (a pass's output is read by the next
+// pass, after more data than the caches hold has gone through them)
+template <typename T>
+__device__ __forceinline__ void store_stream(cpx<T>* p, cpx<T> v) {
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    const V2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+
 // multiply by exp(DIR * i pi / 2): DIR=-1 -> -i, DIR=+1 -> +i
 template <int DIR, typename T>
 __device__ __forceinline__ cpx<T> rotq(cpx<T> a) {

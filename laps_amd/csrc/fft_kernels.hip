@@ -157,7 +157,7 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
     auto lds = [&](int p) { return line + p; };
     auto store = [&](int p, cpx<T> v) {
         const int s = pos_to_stored(p, N, out_cnt);
-        if (valid && s >= 0 && !dead) ob[s * out_ls] = cpx<T>{v.x * scale, v.y * scale};
+        if (valid && s >= 0 && !dead) store_stream(ob + s * out_ls, cpx<T>{v.x * scale, v.y * scale});
     };
     cpx<T>* const stp = stw;
     if constexpr (DUAL) {

@@ -160,7 +160,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, false, ALT>::NT), (FwdCfg<T, N, false
     auto store = [&](int f, int p, cpx<T> val) {
         const int s = pos_to_stored(p, N, a.cy);
         const int my = p <= N / 2 ? p : p - N;
-        if (valid && s >= 0 && my <= ylim && -my <= ylim) ob[f * a.out_fs + s * a.rs] = val;
+        if (valid && s >= 0 && my <= ylim && -my <= ylim) store_stream(ob + f * a.out_fs + s * a.rs, val);
     };
 
     cpx<T> va[I::RMAX], vb[I::RMAX];

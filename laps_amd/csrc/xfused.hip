@@ -259,8 +259,8 @@ __device__ __forceinline__ void r2c_split(const cpx<T> (&v)[4], const cpx<T>* li
         if (k < kx) {
             const cpx<T> zk = v[r];
             const cpx<T> zn = k == 0 ? zk : *(line + P::at(N - k));
-            oa[k] = cpx<T>{h * (zk.x + zn.x), h * (zk.y - zn.y)};
-            if (ob) ob[k] = cpx<T>{h * (zk.y + zn.y), h * (zn.x - zk.x)};
+            store_stream(oa + k, cpx<T>{h * (zk.x + zn.x), h * (zk.y - zn.y)});
+            if (ob) store_stream(ob + k, cpx<T>{h * (zk.y + zn.y), h * (zn.x - zk.x)});
         }
     }
 }
