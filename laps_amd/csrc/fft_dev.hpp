@@ -43,6 +43,14 @@ __device__ __forceinline__ void store_stream(cpx<T>* p, cpx<T> v) {
     __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
 }
 
+// one complex value loaded with the non-temporal hint (read once by its pass)
+template <typename T>
+__device__ __forceinline__ cpx<T> load_stream(const cpx<T>* p) {
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    const V2 w = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+    return cpx<T>{w.x, w.y};
+}
+
 // multiply by exp(DIR * i pi / 2): DIR=-1 -> -i, DIR=+1 -> +i
 template <int DIR, typename T>
 __device__ __forceinline__ cpx<T> rotq(cpx<T> a) {

@@ -134,6 +134,9 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
         const cpx<double> u{double(b2v.x) * k1 - double(b1v.x) * k2, double(b2v.y) * k1 - double(b1v.y) * k2};
         return cpx<T>{T(-u.y), T(u.x)};
     };
+    // every input element is read once, except the B fields that the J blocks read again
+    const bool once = !a.curl || b2 < 4 || b2 == 7;
+    auto rd = [&](int s) -> cpx<TI> { return once ? load_stream(ib + s * in_ls) : ib[s * in_ls]; };
     auto load = [&](int p) -> cpx<T> {
         const int s = pos_to_stored(p, N, in_cnt);
         if (!valid || s < 0 || dead) return cpx<T>{T(0), T(0)};
@@ -147,7 +150,7 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
             }
             return v;
         }
-        const cpx<TI> v = ib[s * in_ls];
+        const cpx<TI> v = rd(s);
         if (ph) {
             const cpx<double> w = cmul(ph[p], wcb);
             return cmul(cpx<T>{T(v.x), T(v.y)}, cpx<T>{T(w.x), T(w.y)});
@@ -175,7 +178,7 @@ __global__ __launch_bounds__((StridedCfg<T, N, P, W>::NT)) void k_strided(const 
                     if (jc >= 0) {
                         x = curl_at(s);
                     } else {
-                        const cpx<TI> q = ib[s * in_ls];
+                        const cpx<TI> q = rd(s);
                         x = cpx<T>{T(q.x), T(q.y)};
                     }
                 }

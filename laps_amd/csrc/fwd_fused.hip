@@ -153,7 +153,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, false, ALT>::NT), (FwdCfg<T, N, false
 
     auto fft_field = [&](int f, cpx<T>* v) {
         const cpx<T>* src = ib + f * a.in_fs;
-        auto load = [&](int p) -> cpx<T> { return valid ? src[p * a.rs] : cpx<T>{T(0), T(0)}; };
+        auto load = [&](int p) -> cpx<T> { return valid ? load_stream(src + p * a.rs) : cpx<T>{T(0), T(0)}; };
         __syncthreads();  // the LDS line may still be read by the previous transform
         return line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, v);
     };
@@ -337,7 +337,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                 if (t >= V.nterm) break;
                 const cpx<T>* src = base + V.t[t].field * a.in_fs;
                 // a column with no kept mode (OPT 1) is not read: the y pass did not store it
-                auto load = [&](int p) -> cpx<T> { return live ? src[p * zls] : cpx<T>{T(0), T(0)}; };
+                auto load = [&](int p) -> cpx<T> { return live ? load_stream(src + p * zls) : cpx<T>{T(0), T(0)}; };
                 __syncthreads();
                 h = line_fft_regs<N, -1, T, typename C::P>(j, stw, load, lds, w);
                 if (h) {
@@ -417,8 +417,8 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                     }
                 }
                 act[q] = k;
-                uq[q] = k ? U[v * a.ufs + mi[q]] : cpx<double>{0.0, 0.0};
-                fq[q] = (k && irk != 0) ? Fp[v * a.ufs + mi[q]] : cpx<T>{T(0), T(0)};
+                uq[q] = k ? load_stream(U + v * a.ufs + mi[q]) : cpx<double>{0.0, 0.0};
+                fq[q] = (k && irk != 0) ? load_stream(Fp + v * a.ufs + mi[q]) : cpx<T>{T(0), T(0)};
             }
 #pragma unroll
             for (int q = 0; q < CH; ++q) {
@@ -448,7 +448,7 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                 }
                 // the RHS of the last substage is never read: the next step's first substage
                 // weighs the previous RHS by dd(0) = 0 and reads none (rktmod.f90:40-42)
-                if (irk != 2) Fp[v * a.ufs + mi[q]] = cpx<T>{T(fnl.x), T(fnl.y)};
+                if (irk != 2) store_stream(Fp + v * a.ufs + mi[q], cpx<T>{T(fnl.x), T(fnl.y)});
                 if (pp.visc && !pp.visc_exp && v >= 1 && v <= 3) {
                     const double den = ts * k2 * pp.nu + 1.0;
                     const double inv = 1.0 / den;
@@ -463,8 +463,8 @@ __global__ __launch_bounds__((FwdCfg<T, N, true>::NT), (FwdCfg<T, N, true>::MINW
                     const double fx = tb.fx[kx], fy = tb.fy[cy], fz = tb.fz[cz];
                     un = {((un.x * fx) * fy) * fz, ((un.y * fx) * fy) * fz};
                 }
-                U[v * a.ufs + mi[q]] = un;
-                if (Uf) Uf[v * a.ufs + mi[q]] = cpx<float>{float(un.x), float(un.y)};
+                store_stream(U + v * a.ufs + mi[q], un);
+                if (Uf) store_stream(Uf + v * a.ufs + mi[q], cpx<float>{float(un.x), float(un.y)});
             }
         }
     }
