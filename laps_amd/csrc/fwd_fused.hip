@@ -31,27 +31,31 @@ namespace laps {
 // (236 registers) and took 2.83 ms; on the low-radix plan 2.74 ms, and 2.58 ms on its
 // variant with 32 busy work-items per line and 16 columns (PlanLowW; the FP64 passes on it
 // need 150-180 registers, one block per CU, or spill).
-// ALT (the FP32 y pass at 256 only): the wide 8·8·4 plan with 16 columns instead of the
+// ALT (the y pass at 256 only): the wide 8·8·4 plan with 16 columns. FP32: instead of the
 // two-stage plan with 32, for launches of one chain (0.82 -> 0.79 ms per mixed 2/3-rule step;
-// with the two chains of phase shift the two-stage plan is faster, 2.47 against 2.69)
+// with the two chains of phase shift the two-stage plan is faster, 2.47 against 2.69). FP64:
+// instead of PlanLow with 8, for launches of two chains, held to 128 registers (92 bytes
+// spilled; 4.56 -> 4.16 ms per phase-shift step; with one chain 1.17 -> 1.18, not used)
 template <typename T, int N, bool ZP, bool ALT = false>
 struct FwdCfg {
-    // the FP32 z pass at 256: the 8·8·4 plan with two last-stage butterflies per work-item
-    // (32 per line, none idle) and 16 columns, 128-byte rows of D, Fp and Uf
-    static constexpr bool WIDE = N == 256 && sizeof(T) == 4 && (ZP || ALT);
+    // the FP32 z pass at 256 and the ALT y passes: the 8·8·4 plan with two last-stage
+    // butterflies per work-item (32 per line, none idle) and 16 columns, 128-byte rows of D,
+    // Fp and Uf in FP32, 256-byte rows of the y pass in FP64
+    static constexpr bool WIDE = N == 256 && (ALT || (sizeof(T) == 4 && ZP));
     using P = std::conditional_t<(sizeof(T) == 4 && N == 256 && !ZP && !ALT), Plan<N>,
                                  std::conditional_t<WIDE, PlanLowW<N>, PlanLow<N>>>;
     using I = PlanInfo<N, P>;
     // columns per block: rows of 128 bytes (8 FP64 or 16 FP32 columns), 256 bytes for the FP32
     // y pass on the two-stage plan; the FP32 z pass has 16 columns only on the wide plan (on
     // PlanLow, 1024-work-item blocks, it measured slower at 256), else 8
-    static constexpr int TX = (sizeof(T) == 8) ? 8 : (WIDE ? 16 : (ZP ? 8 : (N == 256 ? 32 : 16)));
+    static constexpr int TX = (sizeof(T) == 8) ? (WIDE ? 16 : 8) : (WIDE ? 16 : (ZP ? 8 : (N == 256 ? 32 : 16)));
     static constexpr int TPL = I::TPL;
     static constexpr int NT = TX * TPL;
     static constexpr int LS = N + 1;
-    // waves per SIMD the register allocation must allow: the FP32 y pass at 256 (512-thread
-    // blocks, 66 KB of LDS) fits two blocks per CU only at <= 128 VGPRs
-    static constexpr int MINW = (sizeof(T) == 4 && N == 256 && !ZP && !ALT) ? 4 : 1;
+    // waves per SIMD the register allocation must allow: the FP32 y pass at 256 on the
+    // two-stage plan and the FP64 ALT y pass (512-thread blocks, 66-72 KB of LDS) fit two
+    // blocks per CU only at <= 128 VGPRs
+    static constexpr int MINW = (N == 256 && !ZP && (sizeof(T) == 4 ? !ALT : ALT)) ? 4 : 1;
     static_assert(64 % TX == 0 && (NT % 64 == 0 || 64 % NT == 0), "every wave holds whole rows of the tile");
 };
 
@@ -486,8 +490,9 @@ static hipError_t do_yg_c(const cpx<T>* in, cpx<T>* out, const YGArgs& a, const 
 template <typename T, int N>
 static hipError_t do_yg(const cpx<T>* in, cpx<T>* out, const YGArgs& a, const cpx<T>* tw, const double* wxt,
                         const double* wyf, const PhysParams& pp, const Ctl* ctl, int nz, hipStream_t s) {
-    if constexpr (sizeof(T) == 4 && N == 256) {
-        if (a.nch != 2) return do_yg_c<T, N, true>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
+    if constexpr (N == 256) {
+        // the wide plan: FP32 launches of one chain, FP64 launches of two
+        if ((a.nch == 2) == (sizeof(T) == 8)) return do_yg_c<T, N, true>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
     }
     return do_yg_c<T, N, false>(in, out, a, tw, wxt, wyf, pp, ctl, nz, s);
 }
